@@ -1,0 +1,167 @@
+"""Benchmark: chi2Profile evaluations per second on one MI355X.
+
+Workload: ring+envelope radial profile (Nr=1000, Rmin=1e-4", dR=0.002"),
+nxy=4096, dxy=0.01", inc=40 deg, PA=120 deg, dRA=dDec=0.02", Nvis=1e6
+log-uniform baselines in [5e3, 1e7] wavelengths, noisy observation; float64;
+galario's central-pixel guard off (it rejects dR=0.002" at dxy=0.01").
+A step is one chi2Profile evaluation with all inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+
+Prints one JSON line.  `roofline` is measured live: HIP events around each
+stage on the stream the kernels run on give each kernel's average duration;
+`achieved` = that kernel's algorithmic bytes / duration.  `cpu_baseline`
+times the C/OpenMP restatement of galario's CPU loops (FFT by scipy.fft) on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "chi2Profile evals/s fwd & fwd+grad, nxy=4096 Nvis=1e6 FP64; % HBM roofline"
+
+
+def stage_bytes(nxy, nvis, nr, rmax, s=8):
+    """Algorithmic HBM bytes per launch of each stage (each needed array touched once)."""
+    C = nxy // 2 + 1
+    rows = 2 * rmax
+    return {
+        'row_pass': rows * C * 2 * s + nr * s,                   # write the row-transformed box
+        'column_pass': rows * C * 2 * s + nxy * C * 2 * s,       # read it, write the spectrum
+        'sample': 13 * s * nvis,                                 # u,v,Re,Im,w + 4 complex corners
+        'final_sum': 8 * ((nvis + 255) // 256),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--cpu-seconds', type=float, default=12.0, help='CPU-baseline time budget (0: skip)')
+    ap.add_argument('--nvis', type=int, default=1_000_000)
+    ap.add_argument('--nxy', type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU form is a design "
+              f"(see DESIGN.md), not a measured path in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import galario_amd
+    from galario_amd import synthetic as syn
+    from galario_amd import torch_api as ta
+
+    g = galario_amd.double
+    dev = torch.device('cuda:0')
+    torch.cuda.set_device(dev)
+
+    def gpu_model(wl):
+        return g.sampleProfile(wl.intensity, wl.Rmin, wl.dR, wl.nxy, wl.dxy, wl.u, wl.v,
+                               check_central_pixel=False, **wl.kwargs())
+
+    wl = syn.ring_profile_workload(gpu_model, nvis=args.nvis, nxy=args.nxy, seed=1)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)  # noqa: E731
+    I, u, v, re, im, w = map(T, (wl.intensity, wl.u, wl.v, wl.re, wl.im, wl.w))
+    out = torch.empty(1, dtype=torch.float64, device=dev)
+    kw = wl.kwargs()
+
+    def step():
+        ta.chi2_profile(I, wl.Rmin, wl.dR, wl.nxy, wl.dxy, u, v, re, im, w, check_central_pixel=False, out=out, **kw)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    chi2_dev = float(out.item())
+    chi2_host = g.chi2Profile(*wl.args(), check_central_pixel=False, **kw)
+    assert abs(chi2_dev - chi2_host) <= 1e-12 * abs(chi2_host), (chi2_dev, chi2_host)
+    assert not ta.take_oob_flag()
+
+    # timed region: K steps, per-stage HIP events recorded on the kernels' stream
+    ta.stage_timer(True)
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = e0.elapsed_time(e1) / args.steps
+    stage_ms, ncalls = ta.stage_times()
+    ta.stage_timer(False)
+    stage_avg = {k: v_ / max(ncalls, 1) for k, v_ in stage_ms.items()}
+
+    import math
+    rmax = min(int(math.ceil((wl.Rmin + len(wl.intensity) * wl.dR) / wl.dxy)), wl.nxy // 2)  # swept box half side
+    sb = stage_bytes(wl.nxy, len(wl.u), len(wl.intensity), rmax)
+    stages = {k: {'ms': round(stage_avg[k], 5), 'alg_bytes': sb[k],
+                  'GBps': round(sb[k] / (stage_avg[k] * 1e-3) / 1e9, 1) if stage_avg[k] > 0 else None}
+              for k in ta.STAGES}
+    dom = max(('column_pass', 'sample', 'row_pass'), key=lambda k: stage_avg[k])
+    achieved = sb[dom] / (stage_avg[dom] * 1e-3) / 1e9
+    C = wl.nxy // 2 + 1
+    b_fwd = 2 * 8 * wl.nxy * C + 13 * 8 * len(wl.u) + 8 * len(wl.intensity)
+    evals = 1000.0 / ms_per_step
+
+    # host-pointer (drop-in) form, PCIe uploads included: a few calls
+    nh = 5
+    t = time.perf_counter()
+    for _ in range(nh):
+        g.chi2Profile(*wl.args(), check_central_pixel=False, **kw)
+    host_rate = nh / (time.perf_counter() - t)
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        from oracle import port
+        ncores = min(16, os.cpu_count() or 1)
+        port.threads(ncores)
+        times = []
+        tstart = time.perf_counter()
+        while time.perf_counter() - tstart < args.cpu_seconds and len(times) < 60:
+            t = time.perf_counter()
+            c = port.chi2_profile(wl.intensity, wl.Rmin, wl.dR, wl.dxy, wl.nxy, wl.inc, wl.dRA, wl.dDec, wl.duv,
+                                  wl.PA, wl.u, wl.v, wl.re, wl.im, wl.w, workers=ncores)
+            times.append(time.perf_counter() - t)
+        keep = times[1:] if len(times) > 1 else times
+        cpu = {'value': round(1.0 / float(np.mean(keep)), 4), 'unit': 'evals/s', 'cores': ncores, 'kind': 'port',
+               'sample': f'{len(times)} full-size evaluations (first dropped), C/OpenMP restatement of galario '
+                         f'CPU loops + scipy.fft.rfft2(workers={ncores}); chi2 rel. diff vs GPU '
+                         f'{abs(c - chi2_host) / abs(chi2_host):.1e}'}
+
+    res = {
+        'metric': METRIC, 'value': round(evals, 2), 'unit': 'evals/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(ms_per_step, 5), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': f'chi2Profile forward, plain device-resident form, nxy={wl.nxy}, '
+                               f'Nvis={len(wl.u)}, Nr={len(wl.intensity)}, FP64',
+                   'nxy': wl.nxy, 'nvis': len(wl.u), 'nr': len(wl.intensity), 'rmax': rmax,
+                   'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None},
+        'pipeline_roofline': {'alg_bytes_per_eval': b_fwd,
+                              'frac': round(b_fwd * evals / (HBM_PEAK_GBS * 1e9), 4)},
+        'stages': stages,
+        'host_api_evals_per_s_incl_pcie': round(host_rate, 2),
+        'wall_s_timed': round(wall, 3),
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,82 @@
+"""ctypes binding of the C ABI in include/galario_amd.h (libgalario_amd.so).
+
+The library is loaded from this package directory; there is no fallback: if
+it is missing, importing galario_amd fails with an ImportError naming the
+build command.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libgalario_amd.so')
+
+OK, ERR_INVALID, ERR_ALLOC, ERR_RUNTIME = 0, 1, 2, 3
+
+_c_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f'galario_amd: {LIB_PATH} is missing; build it with '
+                          f'`python galario_amd/build.py` (hipcc, gfx950)')
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.galario_last_error.restype = ctypes.c_char_p
+    lib.galario_last_error.argtypes = []
+    lib.galario_init.argtypes = []
+    lib.galario_cleanup.argtypes = []
+    lib.galario_threads.argtypes = [_c_int]
+    lib.galario_threads.restype = _c_int
+    lib.galario_ngpus.argtypes = [_ip]
+    lib.galario_use_gpu.argtypes = [_c_int]
+    lib.galario_free.argtypes = [_vp]
+    lib.galario_free.restype = None
+    lib.galario_take_oob_flag.argtypes = [_ip]
+    lib.galario_stage_timer.argtypes = [_c_int]
+    lib.galario_stage_times.argtypes = [ctypes.POINTER(ctypes.c_double), _c_int, _ip]
+    for suf, real in (('f64', ctypes.c_double), ('f32', ctypes.c_float)):
+        R = real
+        sig = {
+            'sample_profile': [_c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp, _c_int],
+            'sample_image': [_c_int, _c_int, _vp, R, R, R, R, R, _c_int, _vp, _vp, _vp],
+            'chi2_profile': [_c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp, _vp, _vp,
+                             _c_int, _vp],
+            'chi2_image': [_c_int, _c_int, _vp, R, R, R, R, R, _c_int, _vp, _vp, _vp, _vp, _vp, _vp],
+            'sweep': [_c_int, _vp, R, R, _c_int, R, R, _vp, _c_int],
+            'uv_rotate': [R, R, R, _vp, _vp, _c_int, _vp, _vp, _vp, _vp],
+            'copy_input': [_c_int, _c_int, _vp, ctypes.POINTER(_vp)],
+            'fft2d': [_c_int, _c_int, _vp],
+            'fftshift': [_c_int, _c_int, _vp],
+            'fftshift_axis0': [_c_int, _c_int, _vp],
+            'interpolate': [_c_int, _c_int, _vp, R, _c_int, _vp, _vp, R, _vp],
+            'apply_phase_sampled': [R, R, _c_int, _vp, _vp, _vp],
+            'reduce_chi2': [_c_int, _vp, _vp, _vp, _vp, _vp],
+            'chi2_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp, _vp,
+                                 _vp, _c_int, _vp],
+            'sample_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _c_int,
+                                   _vp],
+        }
+        for name, args in sig.items():
+            f = getattr(lib, f'galario_{name}_{suf}')
+            f.argtypes = args
+            f.restype = _c_int
+    return lib
+
+
+lib = _load()
+
+
+def check(status: int) -> None:
+    """Raise the Python exception galario's `except +` mapping gives
+    (python/galario_defs.pxd:22-46, docs/py-api.rst:72-82)."""
+    if status == OK:
+        return
+    msg = lib.galario_last_error().decode(errors='replace')
+    if status == ERR_INVALID:
+        raise ValueError(msg)
+    if status == ERR_ALLOC:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)

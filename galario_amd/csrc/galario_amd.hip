@@ -1,0 +1,987 @@
+// galario_amd: host side of the MI355X HIP implementation of galario's
+// model-to-visibility path, exported through the C ABI in include/galario_amd.h.
+//
+// Per host thread and device there is one Workspace (stream, grow-only device
+// buffers, pinned scalars); per device and size one twiddle table.  A call
+// enqueues: [uploads] -> row pass -> column pass -> sample -> [reduce] ->
+// [downloads + sync].  Argument checks mirror galario's (src/galario.cpp:89-121)
+// and run before anything touches the GPU.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "galario_amd.h"
+#include "gl_kernels.hpp"
+
+using namespace gl;
+
+namespace {
+
+// ------------------------------------------------------------------ errors
+thread_local std::string t_err;
+
+struct Err {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Err{code, msg}; }
+
+[[noreturn]] void invalid(const char* where, const std::string& msg) {
+    fail(GALARIO_ERR_INVALID, std::string("Error in ") + where + ": " + msg);
+}
+
+void hipchk(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+    fail(GALARIO_ERR_RUNTIME, std::string("Error in hip: ") + what + ": " + hipGetErrorString(e));
+}
+#define HC(x) hipchk((x), #x)
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return GALARIO_OK;
+    } catch (const Err& e) {
+        t_err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        t_err = "Error in hip: out of device or host memory";
+        return GALARIO_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        t_err = e.what();
+        return GALARIO_ERR_RUNTIME;
+    } catch (...) {
+        t_err = "unknown error";
+        return GALARIO_ERR_RUNTIME;
+    }
+}
+
+// CHECK_INPUT / CHECK_INPUTXY, src/galario.cpp:101-111
+void check_input(int nx) {
+    if (nx < 2) invalid("check input image", "x dimension = " + std::to_string(nx) + " is less than 2");
+    if (nx % 2 != 0) invalid("check input image", "x dimension = " + std::to_string(nx) + " is odd");
+}
+void check_input_xy(int nx, int ny) {
+    if (nx != ny)
+        invalid("check input image",
+                "Expect a square image but got shape (" + std::to_string(nx) + ", " + std::to_string(ny) + ")");
+    check_input(nx);
+}
+void check_nd(int nd) {
+    if (nd < 0) invalid("check input", "number of (u, v) points = " + std::to_string(nd) + " is negative");
+}
+
+// ------------------------------------------------------------------ memory
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    template <class X>
+    X* get(size_t n) {
+        size_t b = n * sizeof(X);
+        if (b < 256) b = 256;
+        if (b > cap) {
+            release();
+            HC(hipMalloc(&p, b));
+            cap = b;
+        }
+        return static_cast<X*>(p);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct Workspace {
+    int device = -1;
+    hipStream_t own = nullptr;    // stream of the host-pointer entry points
+    hipStream_t last = nullptr;   // stream of the last call that used the buffers
+    bool has_last = false;        // (the null stream is a valid `last`)
+    hipEvent_t ev = nullptr;
+    DevBuf Tm, S, part, scal, flag_host, flag_dev;
+    DevBuf in[8];
+    DevBuf out;
+    double* h_scal = nullptr;     // pinned: [0] result, [1] flag
+    unsigned dev_flag_dirty = 0;
+
+    void release() {
+        Tm.release(); S.release(); part.release(); scal.release();
+        flag_host.release(); flag_dev.release(); out.release();
+        for (auto& b : in) b.release();
+        if (h_scal) (void)hipHostFree(h_scal);
+        if (ev) (void)hipEventDestroy(ev);
+        if (own) (void)hipStreamDestroy(own);
+        h_scal = nullptr; ev = nullptr; own = nullptr; last = nullptr; has_last = false;
+    }
+};
+
+thread_local std::map<int, Workspace*> t_ws;
+
+// ------------------------------------------------------------------ stage timer
+// When enabled, the _dev entry points record a HIP event before each stage and
+// after the last one, on the stream the kernels run on; galario_stage_times
+// synchronises and returns the summed durations per stage.
+enum { ST_ROW = 0, ST_COL = 1, ST_SAMPLE = 2, ST_SUM = 3, NSTAGE = 4 };
+struct StageTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;            // free events
+    std::vector<std::vector<std::pair<int, hipEvent_t>>> calls;   // per call: (stage, event) in order
+    std::vector<std::pair<int, hipEvent_t>> cur;
+    hipEvent_t get() {
+        hipEvent_t e;
+        if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
+        else HC(hipEventCreate(&e));
+        return e;
+    }
+    void mark(int stage, hipStream_t s) {
+        if (!on) return;
+        hipEvent_t e = get();
+        HC(hipEventRecord(e, s));
+        cur.emplace_back(stage, e);
+    }
+    void end_call(hipStream_t s) {
+        if (!on) return;
+        mark(NSTAGE, s);
+        calls.push_back(std::move(cur));
+        cur.clear();
+    }
+};
+thread_local StageTimer t_timer;
+
+
+// Workspace of the current device; orders `s` after the previous user stream.
+Workspace& workspace(hipStream_t s, bool use_own) {
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    Workspace*& w = t_ws[dev];
+    if (!w) {
+        w = new Workspace();
+        w->device = dev;
+        HC(hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking));
+        HC(hipEventCreateWithFlags(&w->ev, hipEventDisableTiming));
+        HC(hipHostMalloc(reinterpret_cast<void**>(&w->h_scal), 64, hipHostMallocDefault));
+        unsigned* fh = w->flag_host.get<unsigned>(1);
+        unsigned* fd = w->flag_dev.get<unsigned>(1);
+        HC(hipMemsetAsync(fh, 0, sizeof(unsigned), w->own));
+        HC(hipMemsetAsync(fd, 0, sizeof(unsigned), w->own));
+        HC(hipStreamSynchronize(w->own));
+    }
+    hipStream_t use = use_own ? w->own : s;
+    if (w->has_last && w->last != use) {
+        HC(hipEventRecord(w->ev, w->last));
+        HC(hipStreamWaitEvent(use, w->ev, 0));
+    }
+    w->last = use;
+    w->has_last = true;
+    return *w;
+}
+
+// ------------------------------------------------------------------ twiddles
+std::mutex g_tw_mu;
+std::map<std::tuple<int, int, int>, void*> g_tw;
+
+// W_N^e = exp(-2 pi i e / N), e in [0, N), in long double, exact on the axes.
+template <typename T>
+const cpx<T>* twiddles(int N) {
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_tw_mu);
+    auto key = std::make_tuple(dev, N, int(sizeof(T)));
+    auto it = g_tw.find(key);
+    if (it != g_tw.end()) return static_cast<const cpx<T>*>(it->second);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    cpx<T>* h = static_cast<cpx<T>*>(std::malloc(sizeof(cpx<T>) * size_t(N)));
+    if (!h) throw std::bad_alloc();
+    for (int e = 0; e < N; ++e) {
+        long double c, s;
+        const long long e4 = 4LL * e;
+        if (e4 % N == 0) {
+            const int q = int(e4 / N);           // quarter turns
+            c = (q == 0) ? 1.0L : (q == 2 ? -1.0L : 0.0L);
+            s = (q == 1) ? 1.0L : (q == 3 ? -1.0L : 0.0L);
+        } else if (2LL * e > N) {                // conj symmetry: W^e = conj(W^{N-e})
+            const long double a = 2.0L * pi * (long double)(N - e) / (long double)N;
+            c = cosl(a);
+            s = -sinl(a);
+        } else {
+            const long double a = 2.0L * pi * (long double)e / (long double)N;
+            c = cosl(a);
+            s = sinl(a);
+        }
+        h[e] = cpx<T>{T(c), T(-s)};
+    }
+    void* d = nullptr;
+    hipError_t err = hipMalloc(&d, sizeof(cpx<T>) * size_t(N));
+    if (err == hipSuccess) err = hipMemcpy(d, h, sizeof(cpx<T>) * size_t(N), hipMemcpyHostToDevice);
+    std::free(h);
+    HC(err);
+    g_tw[key] = d;
+    return static_cast<const cpx<T>*>(d);
+}
+
+// ------------------------------------------------------------------ launches
+template <typename T>
+constexpr int max_fast_log2() { return sizeof(T) == 8 ? 13 : 14; }
+
+// log2(N) when the workgroup-resident FFT covers N, else -1 (generic path).
+template <typename T>
+int fast_log2(int N) {
+    if (N < 128 || (N & (N - 1))) return -1;
+    int L = 0;
+    while ((1 << L) < N) ++L;
+    return L <= max_fast_log2<T>() ? L : -1;
+}
+
+template <typename T, int L, int SRC>
+void launch_rowpass_L(int pairs, hipStream_t s, const RowArgs<T>& a) {
+    hipLaunchKernelGGL((k_rowpass<T, L, SRC>), dim3(pairs), dim3(FftShape<L>::NT), 0, s, a);
+}
+template <typename T, int L, int OUT>
+void launch_colpass_L(int blocks, hipStream_t s, const ColArgs<T>& a) {
+    hipLaunchKernelGGL((k_colpass<T, L, OUT>), dim3(blocks), dim3(FftShape<L>::NT), 0, s, a);
+}
+
+template <typename T, int SRC>
+void launch_rowpass(int L, int pairs, hipStream_t s, const RowArgs<T>& a) {
+    switch (L) {
+        case 7: launch_rowpass_L<T, 7, SRC>(pairs, s, a); break;
+        case 8: launch_rowpass_L<T, 8, SRC>(pairs, s, a); break;
+        case 9: launch_rowpass_L<T, 9, SRC>(pairs, s, a); break;
+        case 10: launch_rowpass_L<T, 10, SRC>(pairs, s, a); break;
+        case 11: launch_rowpass_L<T, 11, SRC>(pairs, s, a); break;
+        case 12: launch_rowpass_L<T, 12, SRC>(pairs, s, a); break;
+        case 13: launch_rowpass_L<T, 13, SRC>(pairs, s, a); break;
+        case 14:
+            if constexpr (sizeof(T) == 4) { launch_rowpass_L<T, 14, SRC>(pairs, s, a); break; }
+            [[fallthrough]];
+        default: fail(GALARIO_ERR_RUNTIME, "internal: no row-pass kernel for this size");
+    }
+}
+
+template <typename T, int OUT>
+void launch_colpass(int L, int blocks, hipStream_t s, const ColArgs<T>& a) {
+    switch (L) {
+        case 7: launch_colpass_L<T, 7, OUT>(blocks, s, a); break;
+        case 8: launch_colpass_L<T, 8, OUT>(blocks, s, a); break;
+        case 9: launch_colpass_L<T, 9, OUT>(blocks, s, a); break;
+        case 10: launch_colpass_L<T, 10, OUT>(blocks, s, a); break;
+        case 11: launch_colpass_L<T, 11, OUT>(blocks, s, a); break;
+        case 12: launch_colpass_L<T, 12, OUT>(blocks, s, a); break;
+        case 13: launch_colpass_L<T, 13, OUT>(blocks, s, a); break;
+        case 14:
+            if constexpr (sizeof(T) == 4) { launch_colpass_L<T, 14, OUT>(blocks, s, a); break; }
+            [[fallthrough]];
+        default: fail(GALARIO_ERR_RUNTIME, "internal: no column-pass kernel for this size");
+    }
+}
+
+inline unsigned blocks_for(size_t n, unsigned per) { return unsigned((n + per - 1) / per); }
+
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 48 * 1024)
+        HC(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(bytes)));
+}
+
+// Shifted (or, for OUT_ROWMAJOR_PLAIN, plain) R2C spectrum of the rows given
+// by `ra` (source fields filled by the caller).  Rs: support half-width.
+template <typename T, int SRC, int OUT>
+cpx<T>* build_spectrum(Workspace& w, hipStream_t s, RowArgs<T> ra, int N, int Rs, cpx<T>* S_out, size_t ldS) {
+    const int C = N / 2 + 1;
+    const int ldT = (C + 3) & ~3;
+    cpx<T>* Tm = w.Tm.get<cpx<T>>(size_t(2 * Rs) * ldT);
+    const cpx<T>* tw = twiddles<T>(N);
+    ra.N = N;
+    ra.Rs = Rs;
+    ra.ldT = ldT;
+    ra.tw = tw;
+    ra.Tm = Tm;
+    ColArgs<T> ca{Tm, N, Rs, ldT, tw, S_out, ldS};
+    const int L = fast_log2<T>(N);
+    if (L > 0) {
+        t_timer.mark(ST_ROW, s);
+        launch_rowpass<T, SRC>(L, Rs, s, ra);
+        t_timer.mark(ST_COL, s);
+        launch_colpass<T, OUT>(L, 8 * ((C + 7) / 8), s, ca);
+    } else {
+        const size_t lds_row = sizeof(T) * size_t(2 * Rs);
+        if (lds_row > 160 * 1024) fail(GALARIO_ERR_INVALID, "Error in fft: image size not supported");
+        allow_lds(k_rowdft_generic<T, SRC>, lds_row);
+        hipLaunchKernelGGL((k_rowdft_generic<T, SRC>), dim3(2 * Rs), dim3(256), lds_row, s, ra);
+        const size_t lds_col = sizeof(cpx<T>) * size_t(2 * Rs);
+        if (lds_col > 160 * 1024) fail(GALARIO_ERR_INVALID, "Error in fft: image size not supported");
+        allow_lds(k_coldft_generic<T, OUT>, lds_col);
+        hipLaunchKernelGGL((k_coldft_generic<T, OUT>), dim3(C), dim3(256), lds_col, s, ca);
+    }
+    HC(hipGetLastError());
+    return S_out;
+}
+
+// ------------------------------------------------------------------ setup of a profile evaluation
+template <typename T>
+struct ProfileSetup {
+    SweepPrm<T> sw;
+    int rmax;
+};
+
+// Validation and derived scalars of create_image_h (src/galario.cpp:1186-1235),
+// in dreal arithmetic as the reference computes them.
+template <typename T>
+ProfileSetup<T> setup_profile(int nr, T Rmin, T dR, T dxy, int nxy, T inc, int check_central) {
+    check_input(nxy);
+    if (!(dR > T(0)) || !(dxy > T(0)) || !std::isfinite(double(dR)) || !std::isfinite(double(dxy)))
+        invalid("create image", "expect dR > 0 and dxy > 0");
+    if (check_central) {
+        const T ratio = T((double(dxy) / 2. - double(Rmin)) / double(dR));
+        if (ratio < 5)
+            invalid("create image",
+                    "Expect (dxy/2-Rmin)/dR > 5, i.e. dR small enough to ensure reliable interpolation the inside "
+                    "central pixel but got (dxy/2-Rmin)/dR = " +
+                        std::to_string(ratio) + ". Try reducing dR.");
+    }
+    const double fin = std::floor(double(T(T(dxy / T(2)) - Rmin) / dR));
+    if (!(fin >= 0.0) || !(fin + 1.0 < double(nr)))
+        invalid("create image", "the central pixel needs intensity[floor((dxy/2-Rmin)/dR) + 1] but got " +
+                                    std::to_string(nr) + " intensity values and floor((dxy/2-Rmin)/dR) = " +
+                                    std::to_string(fin));
+    ProfileSetup<T> ps{};
+    ps.sw.nr = nr;
+    ps.sw.Rmin = Rmin;
+    ps.sw.dR = dR;
+    ps.sw.dxy = dxy;
+    ps.sw.cos_inc = T(std::cos(double(inc)));
+    ps.sw.sr_to_px = dxy * dxy;
+    ps.sw.iIN = int(fin);
+    const double rm = std::ceil(double((Rmin + T(nr) * dR) / dxy));
+    int rmax = rm > double(nxy / 2) ? nxy / 2 : (rm < 0.0 ? 0 : int(rm));
+    ps.sw.rmax = rmax;
+    ps.rmax = rmax;
+    return ps;
+}
+
+template <typename T>
+struct SampleSetup {
+    int rotate;
+    T cosPA, sinPA;
+    int phase;
+    T dRA2, dDec2;
+    T dRArot, dDecrot;
+};
+
+// uv_rotate_h (src/galario.cpp:976-999) scalars and apply_phase_sampled_h's
+// (908-921) skip test and 2*pi scaling, in dreal arithmetic.
+template <typename T>
+SampleSetup<T> setup_sample(T PA, T dRA, T dDec) {
+    SampleSetup<T> s{};
+    if (PA == T(0)) {
+        s.rotate = 0;
+        s.cosPA = T(1);
+        s.sinPA = T(0);
+        s.dRArot = dRA;
+        s.dDecrot = dDec;
+    } else {
+        s.rotate = 1;
+        s.cosPA = T(std::cos(double(PA)));
+        s.sinPA = T(std::sin(double(PA)));
+        s.dRArot = dRA * s.cosPA - dDec * s.sinPA;
+        s.dDecrot = dRA * s.sinPA + dDec * s.cosPA;
+    }
+    s.phase = !(s.dRArot == T(0) && s.dDecrot == T(0));
+    s.dRA2 = T(double(s.dRArot) * (2. * double(T(M_PI))));
+    s.dDec2 = T(double(s.dDecrot) * (2. * double(T(M_PI))));
+    return s;
+}
+
+template <typename T>
+SampleArgs<T> sample_args(const cpx<T>* S, long long rs, long long cs, int nrow, int ncol, T v_origin, T duv,
+                          const SampleSetup<T>& ss, int nd, const T* u, const T* v) {
+    SampleArgs<T> a{};
+    a.S = S;
+    a.rs = rs;
+    a.cs = cs;
+    a.nrow = nrow;
+    a.ncol = ncol;
+    a.v_origin = v_origin;
+    a.duv = duv;
+    a.rotate = ss.rotate;
+    a.cosPA = ss.cosPA;
+    a.sinPA = ss.sinPA;
+    a.phase = ss.phase;
+    a.dRA2 = ss.dRA2;
+    a.dDec2 = ss.dDec2;
+    a.nd = nd;
+    a.u = u;
+    a.v = v;
+    return a;
+}
+
+// chi2 of sample args `a` into chi2_d[0] (double) and, if given, chi2_t[0].
+template <typename T>
+void launch_chi2(Workspace& w, hipStream_t s, SampleArgs<T> a, const T* re, const T* im, const T* wt,
+                 double* chi2_d, T* chi2_t) {
+    const unsigned nb = blocks_for(size_t(a.nd), 256);
+    double* part = w.part.get<double>(nb ? nb : 1);
+    a.re = re;
+    a.im = im;
+    a.w = wt;
+    a.partials = part;
+    t_timer.mark(ST_SAMPLE, s);
+    if (nb) hipLaunchKernelGGL((k_sample<T, MODE_CHI2>), dim3(nb), dim3(256), 0, s, a);
+    t_timer.mark(ST_SUM, s);
+    hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, part, int(nb), chi2_d, chi2_t);
+    HC(hipGetLastError());
+}
+
+template <typename T>
+void launch_vis(hipStream_t s, SampleArgs<T> a, cpx<T>* vis) {
+    a.vis = vis;
+    if (a.nd) hipLaunchKernelGGL((k_sample<T, MODE_VIS>), dim3(blocks_for(size_t(a.nd), 256)), dim3(256), 0, s, a);
+    HC(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ host transfer helpers
+template <typename X>
+X* upload(DevBuf& b, const X* h, size_t n, hipStream_t s) {
+    X* d = b.get<X>(n);
+    if (n) HC(hipMemcpyAsync(d, h, n * sizeof(X), hipMemcpyHostToDevice, s));
+    return d;
+}
+
+template <typename X>
+void download(X* h, const X* d, size_t n, hipStream_t s) {
+    if (n) HC(hipMemcpyAsync(h, d, n * sizeof(X), hipMemcpyDeviceToHost, s));
+}
+
+// Synchronise the own stream and raise if a point fell outside the grid.
+void finish_host_call(Workspace& w, hipStream_t s, const unsigned* flag_d) {
+    unsigned* hf = reinterpret_cast<unsigned*>(w.h_scal + 1);
+    HC(hipMemcpyAsync(hf, flag_d, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    if (*hf)
+        invalid("interpolate",
+                "at least one (u, v) point falls outside the Fourier grid (|u|/duv must not exceed ncol-1 and "
+                "nrow/2 + v/duv must lie in [0, nrow-1]); increase nxy or reduce dxy");
+}
+
+unsigned* reset_host_flag(Workspace& w, hipStream_t s) {
+    unsigned* f = w.flag_host.get<unsigned>(1);
+    HC(hipMemsetAsync(f, 0, sizeof(unsigned), s));
+    return f;
+}
+
+// ------------------------------------------------------------------ main functions (host pointers)
+template <typename T>
+int sample_profile_host(int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec, T duv, T PA,
+                        int nd, const T* u, const T* v, T* vis, int check_central) {
+    return guard([&] {
+        check_nd(nd);
+        ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
+        SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        unsigned* flag = reset_host_flag(w, s);
+        ps.sw.I = upload(w.in[0], I, size_t(nr), s);
+        const T* du = upload(w.in[1], u, size_t(nd), s);
+        const T* dv = upload(w.in[2], v, size_t(nd), s);
+        const int N = nxy, C = N / 2 + 1;
+        RowArgs<T> ra{};
+        ra.sw = ps.sw;
+        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
+        build_spectrum<T, SRC_SWEEP, OUT_COLMAJOR_SHIFT>(w, s, ra, N, ps.rmax > 0 ? ps.rmax : 1, S, size_t(N));
+        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, T(1), duv, ss, nd, du, dv);
+        sa.flag = flag;
+        cpx<T>* dvis = w.out.get<cpx<T>>(size_t(nd));
+        launch_vis(s, sa, dvis);
+        download(reinterpret_cast<cpx<T>*>(vis), dvis, size_t(nd), s);
+        finish_host_call(w, s, flag);
+    });
+}
+
+template <typename T>
+int chi2_profile_host(int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec, T duv, T PA, int nd,
+                      const T* u, const T* v, const T* re, const T* im, const T* wt, int check_central, T* chi2) {
+    return guard([&] {
+        check_nd(nd);
+        ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
+        SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        unsigned* flag = reset_host_flag(w, s);
+        ps.sw.I = upload(w.in[0], I, size_t(nr), s);
+        const T* du = upload(w.in[1], u, size_t(nd), s);
+        const T* dv = upload(w.in[2], v, size_t(nd), s);
+        const T* dre = upload(w.in[3], re, size_t(nd), s);
+        const T* dim = upload(w.in[4], im, size_t(nd), s);
+        const T* dw = upload(w.in[5], wt, size_t(nd), s);
+        const int N = nxy, C = N / 2 + 1;
+        RowArgs<T> ra{};
+        ra.sw = ps.sw;
+        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
+        build_spectrum<T, SRC_SWEEP, OUT_COLMAJOR_SHIFT>(w, s, ra, N, ps.rmax > 0 ? ps.rmax : 1, S, size_t(N));
+        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, T(1), duv, ss, nd, du, dv);
+        sa.flag = flag;
+        double* res = w.scal.get<double>(4);
+        launch_chi2<T>(w, s, sa, dre, dim, dw, res, nullptr);
+        download(w.h_scal, res, 1, s);
+        finish_host_call(w, s, flag);
+        *chi2 = T(w.h_scal[0]);
+    });
+}
+
+template <typename T>
+int sample_image_host(int nx, int ny, const T* image, T v_origin, T dRA, T dDec, T duv, T PA, int nd, const T* u,
+                      const T* v, T* vis) {
+    return guard([&] {
+        check_nd(nd);
+        check_input_xy(nx, ny);
+        SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        unsigned* flag = reset_host_flag(w, s);
+        const int N = nx, C = N / 2 + 1;
+        RowArgs<T> ra{};
+        ra.img = upload(w.in[0], image, size_t(N) * N, s);
+        ra.ldimg = N;
+        const T* du = upload(w.in[1], u, size_t(nd), s);
+        const T* dv = upload(w.in[2], v, size_t(nd), s);
+        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
+        build_spectrum<T, SRC_IMAGE_SHIFT, OUT_COLMAJOR_SHIFT>(w, s, ra, N, N / 2, S, size_t(N));
+        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, v_origin, duv, ss, nd, du, dv);
+        sa.flag = flag;
+        cpx<T>* dvis = w.out.get<cpx<T>>(size_t(nd));
+        launch_vis(s, sa, dvis);
+        download(reinterpret_cast<cpx<T>*>(vis), dvis, size_t(nd), s);
+        finish_host_call(w, s, flag);
+    });
+}
+
+template <typename T>
+int chi2_image_host(int nx, int ny, const T* image, T v_origin, T dRA, T dDec, T duv, T PA, int nd, const T* u,
+                    const T* v, const T* re, const T* im, const T* wt, T* chi2) {
+    return guard([&] {
+        check_nd(nd);
+        check_input_xy(nx, ny);
+        SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        unsigned* flag = reset_host_flag(w, s);
+        const int N = nx, C = N / 2 + 1;
+        RowArgs<T> ra{};
+        ra.img = upload(w.in[0], image, size_t(N) * N, s);
+        ra.ldimg = N;
+        const T* du = upload(w.in[1], u, size_t(nd), s);
+        const T* dv = upload(w.in[2], v, size_t(nd), s);
+        const T* dre = upload(w.in[3], re, size_t(nd), s);
+        const T* dim = upload(w.in[4], im, size_t(nd), s);
+        const T* dw = upload(w.in[5], wt, size_t(nd), s);
+        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
+        build_spectrum<T, SRC_IMAGE_SHIFT, OUT_COLMAJOR_SHIFT>(w, s, ra, N, N / 2, S, size_t(N));
+        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, v_origin, duv, ss, nd, du, dv);
+        sa.flag = flag;
+        double* res = w.scal.get<double>(4);
+        launch_chi2<T>(w, s, sa, dre, dim, dw, res, nullptr);
+        download(w.h_scal, res, 1, s);
+        finish_host_call(w, s, flag);
+        *chi2 = T(w.h_scal[0]);
+    });
+}
+
+template <typename T>
+int sweep_host(int nr, const T* I, T Rmin, T dR, int nxy, T dxy, T inc, T* image, int check_central) {
+    return guard([&] {
+        ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        ps.sw.I = upload(w.in[0], I, size_t(nr), s);
+        const int ld = 2 * (nxy / 2 + 1);
+        T* dimg = w.out.get<T>(size_t(nxy) * ld);
+        hipLaunchKernelGGL((k_sweep_image<T>), dim3(blocks_for(size_t(nxy) * ld, 256)), dim3(256), 0, s, ps.sw, nxy,
+                           ld, dimg);
+        HC(hipGetLastError());
+        download(image, dimg, size_t(nxy) * ld, s);
+        HC(hipStreamSynchronize(s));
+    });
+}
+
+template <typename T>
+int uv_rotate_host(T PA, T dRA, T dDec, T* dRArot, T* dDecrot, int nd, const T* u, const T* v, T* urot, T* vrot) {
+    return guard([&] {
+        check_nd(nd);
+        SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        const T* du = upload(w.in[1], u, size_t(nd), s);
+        const T* dv = upload(w.in[2], v, size_t(nd), s);
+        T* dur = w.in[3].get<T>(size_t(nd));
+        T* dvr = w.in[4].get<T>(size_t(nd));
+        if (ss.rotate) {
+            if (nd)
+                hipLaunchKernelGGL((k_rotate<T>), dim3(blocks_for(size_t(nd), 256)), dim3(256), 0, s, nd, ss.cosPA,
+                                   ss.sinPA, du, dv, dur, dvr);
+            HC(hipGetLastError());
+        } else if (nd) {
+            HC(hipMemcpyAsync(dur, du, size_t(nd) * sizeof(T), hipMemcpyDeviceToDevice, s));
+            HC(hipMemcpyAsync(dvr, dv, size_t(nd) * sizeof(T), hipMemcpyDeviceToDevice, s));
+        }
+        download(urot, dur, size_t(nd), s);
+        download(vrot, dvr, size_t(nd), s);
+        HC(hipStreamSynchronize(s));
+        *dRArot = ss.dRArot;
+        *dDecrot = ss.dDecrot;
+    });
+}
+
+template <typename T>
+int copy_input_host(int nx, int ny, const T* realdata, void** out) {
+    return guard([&] {
+        check_input_xy(nx, ny);
+        const size_t ld = size_t(2 * (ny / 2 + 1));
+        T* buf = static_cast<T*>(std::calloc(size_t(nx) * ld, sizeof(T)));
+        if (!buf) throw std::bad_alloc();
+        for (int i = 0; i < nx; ++i) std::memcpy(buf + size_t(i) * ld, realdata + size_t(i) * ny, sizeof(T) * ny);
+        *out = buf;
+    });
+}
+
+template <typename T>
+int fft2d_host(int nx, int ny, void* data) {
+    return guard([&] {
+        check_input_xy(nx, ny);
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        const int N = nx, C = N / 2 + 1, ld = 2 * C;
+        RowArgs<T> ra{};
+        ra.img = upload(w.in[0], static_cast<const T*>(data), size_t(N) * ld, s);
+        ra.ldimg = ld;
+        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
+        build_spectrum<T, SRC_IMAGE_PLAIN, OUT_ROWMAJOR_PLAIN>(w, s, ra, N, N / 2, S, size_t(C));
+        download(static_cast<cpx<T>*>(data), S, size_t(C) * N, s);
+        HC(hipStreamSynchronize(s));
+    });
+}
+
+template <typename T>
+int fftshift_host(int nx, int ny, void* data) {
+    return guard([&] {
+        check_input_xy(nx, ny);
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        const int N = nx, ld = 2 * (N / 2 + 1);
+        const size_t n = size_t(N) * ld;
+        const T* din = upload(w.in[0], static_cast<const T*>(data), n, s);
+        T* dout = w.out.get<T>(n);
+        hipLaunchKernelGGL((k_fftshift_padded<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, N, ld, din, dout);
+        HC(hipGetLastError());
+        download(static_cast<T*>(data), dout, n, s);
+        HC(hipStreamSynchronize(s));
+    });
+}
+
+template <typename T>
+int fftshift_axis0_host(int nrow, int ncol, void* data) {
+    return guard([&] {
+        check_input(nrow);
+        if (ncol < 0) invalid("check input", "ncol = " + std::to_string(ncol) + " is negative");
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        const size_t n = size_t(nrow) * ncol;
+        cpx<T>* d = upload(w.in[0], static_cast<const cpx<T>*>(data), n, s);
+        const size_t half = size_t(nrow / 2) * ncol;
+        if (half)
+            hipLaunchKernelGGL((k_fftshift_axis0<T>), dim3(blocks_for(half, 256)), dim3(256), 0, s, nrow, ncol, d);
+        HC(hipGetLastError());
+        download(static_cast<cpx<T>*>(data), d, n, s);
+        HC(hipStreamSynchronize(s));
+    });
+}
+
+template <typename T>
+int interpolate_host(int nrow, int ncol, const void* data, T v_origin, int nd, const T* u, const T* v, T duv,
+                     void* vis) {
+    return guard([&] {
+        check_nd(nd);
+        if (nrow < 2 || ncol < 2)
+            invalid("interpolate", "expect a matrix of at least 2 x 2 but got (" + std::to_string(nrow) + ", " +
+                                       std::to_string(ncol) + ")");
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        unsigned* flag = reset_host_flag(w, s);
+        const cpx<T>* dS = upload(w.in[0], static_cast<const cpx<T>*>(data), size_t(nrow) * ncol, s);
+        const T* du = upload(w.in[1], u, size_t(nd), s);
+        const T* dv = upload(w.in[2], v, size_t(nd), s);
+        SampleSetup<T> ss{};   // no rotation, no phase
+        SampleArgs<T> sa = sample_args<T>(dS, ncol, 1, nrow, ncol, v_origin, duv, ss, nd, du, dv);
+        sa.flag = flag;
+        cpx<T>* dvis = w.out.get<cpx<T>>(size_t(nd));
+        launch_vis(s, sa, dvis);
+        download(static_cast<cpx<T>*>(vis), dvis, size_t(nd), s);
+        finish_host_call(w, s, flag);
+    });
+}
+
+template <typename T>
+int apply_phase_host(T dRA, T dDec, int nd, const T* u, const T* v, void* vis) {
+    return guard([&] {
+        check_nd(nd);
+        if (dRA == T(0) && dDec == T(0)) return;   // apply_phase_sampled_h returns early
+        const T dRA2 = T(double(dRA) * (2. * double(T(M_PI))));
+        const T dDec2 = T(double(dDec) * (2. * double(T(M_PI))));
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        const T* du = upload(w.in[1], u, size_t(nd), s);
+        const T* dv = upload(w.in[2], v, size_t(nd), s);
+        cpx<T>* dvis = upload(w.out, static_cast<const cpx<T>*>(vis), size_t(nd), s);
+        if (nd)
+            hipLaunchKernelGGL((k_apply_phase<T>), dim3(blocks_for(size_t(nd), 256)), dim3(256), 0, s, nd, dRA2,
+                               dDec2, du, dv, dvis);
+        HC(hipGetLastError());
+        download(static_cast<cpx<T>*>(vis), dvis, size_t(nd), s);
+        HC(hipStreamSynchronize(s));
+    });
+}
+
+template <typename T>
+int reduce_chi2_host(int nd, const T* re, const T* im, const void* vis, const T* wt, T* chi2) {
+    return guard([&] {
+        check_nd(nd);
+        Workspace& w = workspace(nullptr, true);
+        hipStream_t s = w.own;
+        const T* dre = upload(w.in[3], re, size_t(nd), s);
+        const T* dim = upload(w.in[4], im, size_t(nd), s);
+        const T* dw = upload(w.in[5], wt, size_t(nd), s);
+        const cpx<T>* dvis = upload(w.out, static_cast<const cpx<T>*>(vis), size_t(nd), s);
+        const unsigned nb = blocks_for(size_t(nd), 256);
+        double* part = w.part.get<double>(nb ? nb : 1);
+        double* res = w.scal.get<double>(4);
+        if (nb) hipLaunchKernelGGL((k_chi2_terms<T>), dim3(nb), dim3(256), 0, s, nd, dre, dim, dvis, dw, part);
+        hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, part, int(nb), res, (T*)nullptr);
+        HC(hipGetLastError());
+        download(w.h_scal, res, 1, s);
+        HC(hipStreamSynchronize(s));
+        *chi2 = T(w.h_scal[0]);
+    });
+}
+
+// ------------------------------------------------------------------ device-resident forms
+template <typename T>
+int chi2_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec, T duv,
+                     T PA, int nd, const T* u, const T* v, const T* re, const T* im, const T* wt, int check_central,
+                     double* chi2_d) {
+    return guard([&] {
+        check_nd(nd);
+        ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
+        SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        Workspace& w = workspace(s, false);
+        ps.sw.I = I;
+        const int N = nxy, C = N / 2 + 1;
+        RowArgs<T> ra{};
+        ra.sw = ps.sw;
+        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
+        build_spectrum<T, SRC_SWEEP, OUT_COLMAJOR_SHIFT>(w, s, ra, N, ps.rmax > 0 ? ps.rmax : 1, S, size_t(N));
+        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, T(1), duv, ss, nd, u, v);
+        sa.flag = w.flag_dev.get<unsigned>(1);
+        launch_chi2<T>(w, s, sa, re, im, wt, chi2_d, nullptr);
+        t_timer.end_call(s);
+    });
+}
+
+template <typename T>
+int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec, T duv,
+                       T PA, int nd, const T* u, const T* v, int check_central, void* vis_d) {
+    return guard([&] {
+        check_nd(nd);
+        ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
+        SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        Workspace& w = workspace(s, false);
+        ps.sw.I = I;
+        const int N = nxy, C = N / 2 + 1;
+        RowArgs<T> ra{};
+        ra.sw = ps.sw;
+        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
+        build_spectrum<T, SRC_SWEEP, OUT_COLMAJOR_SHIFT>(w, s, ra, N, ps.rmax > 0 ? ps.rmax : 1, S, size_t(N));
+        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, T(1), duv, ss, nd, u, v);
+        sa.flag = w.flag_dev.get<unsigned>(1);
+        launch_vis(s, sa, static_cast<cpx<T>*>(vis_d));
+    });
+}
+
+int g_threads = 16;
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* galario_last_error(void) { return t_err.c_str(); }
+
+int galario_init(void) {
+    return guard([] {});
+}
+
+int galario_cleanup(void) {
+    return guard([] {
+        for (auto& kv : t_ws) {
+            if (kv.second) {
+                int cur = 0;
+                HC(hipGetDevice(&cur));
+                HC(hipSetDevice(kv.first));
+                (void)hipDeviceSynchronize();
+                kv.second->release();
+                delete kv.second;
+                HC(hipSetDevice(cur));
+            }
+        }
+        t_ws.clear();
+        std::lock_guard<std::mutex> lock(g_tw_mu);
+        for (auto& kv : g_tw) (void)hipFree(kv.second);
+        g_tw.clear();
+    });
+}
+
+int galario_threads(int num) {
+    if (num > 0) g_threads = num;
+    return g_threads;
+}
+
+int galario_ngpus(int* out) {
+    return guard([&] {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e == hipErrorNoDevice) n = 0;
+        else HC(e);
+        *out = n;
+    });
+}
+
+int galario_use_gpu(int device_id) {
+    return guard([&] { HC(hipSetDevice(device_id)); });
+}
+
+void galario_free(void* ptr) { std::free(ptr); }
+
+#define GL_BOTH(MACRO) MACRO(f64, double) MACRO(f32, float)
+
+#define GL_MAIN(SUF, T)                                                                                           \
+    int galario_sample_profile_##SUF(int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec,     \
+                                     T duv, T PA, int nd, const T* u, const T* v, T* vis, int cc) {              \
+        return sample_profile_host<T>(nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v, vis, cc);    \
+    }                                                                                                             \
+    int galario_sample_image_##SUF(int nx, int ny, const T* image, T vo, T dRA, T dDec, T duv, T PA, int nd,     \
+                                   const T* u, const T* v, T* vis) {                                             \
+        return sample_image_host<T>(nx, ny, image, vo, dRA, dDec, duv, PA, nd, u, v, vis);                       \
+    }                                                                                                             \
+    int galario_chi2_profile_##SUF(int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec,       \
+                                   T duv, T PA, int nd, const T* u, const T* v, const T* re, const T* im,        \
+                                   const T* w, int cc, T* chi2) {                                                \
+        return chi2_profile_host<T>(nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v, re, im, w, cc, \
+                                    chi2);                                                                        \
+    }                                                                                                             \
+    int galario_chi2_image_##SUF(int nx, int ny, const T* image, T vo, T dRA, T dDec, T duv, T PA, int nd,       \
+                                 const T* u, const T* v, const T* re, const T* im, const T* w, T* chi2) {        \
+        return chi2_image_host<T>(nx, ny, image, vo, dRA, dDec, duv, PA, nd, u, v, re, im, w, chi2);             \
+    }                                                                                                             \
+    int galario_sweep_##SUF(int nr, const T* I, T Rmin, T dR, int nxy, T dxy, T inc, T* image, int cc) {         \
+        return sweep_host<T>(nr, I, Rmin, dR, nxy, dxy, inc, image, cc);                                         \
+    }                                                                                                             \
+    int galario_uv_rotate_##SUF(T PA, T dRA, T dDec, T* dRArot, T* dDecrot, int nd, const T* u, const T* v,      \
+                                T* urot, T* vrot) {                                                              \
+        return uv_rotate_host<T>(PA, dRA, dDec, dRArot, dDecrot, nd, u, v, urot, vrot);                          \
+    }                                                                                                             \
+    int galario_copy_input_##SUF(int nx, int ny, const T* realdata, void** out) {                                \
+        return copy_input_host<T>(nx, ny, realdata, out);                                                        \
+    }                                                                                                             \
+    int galario_fft2d_##SUF(int nx, int ny, void* data) { return fft2d_host<T>(nx, ny, data); }                  \
+    int galario_fftshift_##SUF(int nx, int ny, void* data) { return fftshift_host<T>(nx, ny, data); }            \
+    int galario_fftshift_axis0_##SUF(int nrow, int ncol, void* data) {                                           \
+        return fftshift_axis0_host<T>(nrow, ncol, data);                                                         \
+    }                                                                                                             \
+    int galario_interpolate_##SUF(int nrow, int ncol, const void* data, T vo, int nd, const T* u, const T* v,    \
+                                  T duv, void* vis) {                                                            \
+        return interpolate_host<T>(nrow, ncol, data, vo, nd, u, v, duv, vis);                                    \
+    }                                                                                                             \
+    int galario_apply_phase_sampled_##SUF(T dRA, T dDec, int nd, const T* u, const T* v, void* vis) {            \
+        return apply_phase_host<T>(dRA, dDec, nd, u, v, vis);                                                    \
+    }                                                                                                             \
+    int galario_reduce_chi2_##SUF(int nd, const T* re, const T* im, const void* vis, const T* w, T* chi2) {      \
+        return reduce_chi2_host<T>(nd, re, im, vis, w, chi2);                                                    \
+    }                                                                                                             \
+    int galario_chi2_profile_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc,    \
+                                       T dRA, T dDec, T duv, T PA, int nd, const T* u, const T* v, const T* re,  \
+                                       const T* im, const T* w, int cc, double* chi2_d) {                        \
+        return chi2_profile_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v, re, im, \
+                                   w, cc, chi2_d);                                                               \
+    }                                                                                                             \
+    int galario_sample_profile_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc,  \
+                                         T dRA, T dDec, T duv, T PA, int nd, const T* u, const T* v, int cc,     \
+                                         void* vis_d) {                                                          \
+        return sample_profile_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v, cc,   \
+                                     vis_d);                                                                      \
+    }
+
+GL_BOTH(GL_MAIN)
+
+int galario_stage_timer(int enable) {
+    return guard([&] {
+        t_timer.on = enable != 0;
+        for (auto& c : t_timer.calls)
+            for (auto& pe : c) t_timer.pool.push_back(pe.second);
+        t_timer.calls.clear();
+        for (auto& pe : t_timer.cur) t_timer.pool.push_back(pe.second);
+        t_timer.cur.clear();
+    });
+}
+
+int galario_stage_times(double* ms, int nstage, int* ncalls) {
+    return guard([&] {
+        std::vector<double> acc(NSTAGE, 0.0);
+        for (auto& c : t_timer.calls) {
+            for (size_t k = 0; k + 1 < c.size(); ++k) {
+                HC(hipEventSynchronize(c[k + 1].second));
+                float t = 0.f;
+                HC(hipEventElapsedTime(&t, c[k].second, c[k + 1].second));
+                acc[size_t(c[k].first)] += double(t);
+            }
+        }
+        for (int k = 0; k < nstage && k < NSTAGE; ++k) ms[k] = acc[size_t(k)];
+        *ncalls = int(t_timer.calls.size());
+    });
+}
+
+int galario_take_oob_flag(int* out) {
+    return guard([&] {
+        int dev = 0;
+        HC(hipGetDevice(&dev));
+        auto it = t_ws.find(dev);
+        if (it == t_ws.end() || !it->second) {
+            *out = 0;
+            return;
+        }
+        Workspace& w = *it->second;
+        unsigned* fd = w.flag_dev.get<unsigned>(1);
+        hipStream_t s = w.has_last ? w.last : w.own;
+        unsigned* hf = reinterpret_cast<unsigned*>(w.h_scal + 1);
+        HC(hipMemcpyAsync(hf, fd, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        HC(hipMemsetAsync(fd, 0, sizeof(unsigned), s));
+        HC(hipStreamSynchronize(s));
+        *out = *hf ? 1 : 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,467 @@
+// Pipeline kernels: profile/image -> shifted R2C spectrum -> sampled visibilities -> chi2.
+//
+// Data layout in HBM (one evaluation):
+//   Tm  : row-transformed image, only the 2*Rs rows that can be non-zero,
+//         row-major [2*Rs][ldT] complex (ldT >= N/2+1).  Compact row kk maps
+//         to quadrant-shifted image row kk (kk < Rs) or N-2*Rs+kk (kk >= Rs).
+//   S   : shifted spectrum, column-major [N/2+1][ldS] complex: element
+//         (v-row r, u-column l) at l*ldS + r, r = (k + N/2) mod N for FFT row
+//         k (the reference's fftshift_axis0 folded into the stores).
+//         The four interpolation corners are then two 32-byte pairs.
+//
+// All stages that restate galario arithmetic (sweep, central pixel,
+// interpolation, phase, chi2 terms) are compiled without FMA contraction so
+// they round exactly as the reference's -std=c++11 x86-64 build.
+#pragma once
+
+#include <type_traits>
+
+#include "gl_common.hpp"
+#include "gl_fft.hpp"
+
+namespace gl {
+
+enum { SRC_SWEEP = 0, SRC_IMAGE_SHIFT = 1, SRC_IMAGE_PLAIN = 2 };
+enum { OUT_COLMAJOR_SHIFT = 0, OUT_ROWMAJOR_PLAIN = 1 };
+enum { MODE_VIS = 0, MODE_CHI2 = 1 };
+
+// ---------------------------------------------------------------------------
+// sweep (galario.cpp:1048-1071) and central pixel (galario.cpp:1214-1232)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SweepPrm {
+    const T* I;        // radial profile, nr values (device)
+    int nr;
+    T Rmin, dR, dxy, cos_inc, sr_to_px;
+    int iIN;           // floor((dxy/2 - Rmin)/dR), validated on the host
+    int rmax;          // half side of the swept box (0: only the central pixel)
+};
+
+// Pixel value at x = xi*dxy, y = yi*dxy (xi = N/2 - column, yi = N/2 - row).
+template <typename T>
+__device__ __forceinline__ T sweep_pixel(int xi, int yi, const SweepPrm<T>& p) {
+    const T x = T(xi) * p.dxy;
+    const T y = T(yi) * p.dxy;
+    T r;
+    if constexpr (std::is_same<T, double>::value) {
+        const double xs = x / p.cos_inc;
+        r = sqrt(xs * xs + y * y);
+    } else {
+        // pow(float, 2) and the sum are double in the reference; r is stored as float
+        const double xs = double(x / p.cos_inc);
+        r = float(sqrt(xs * xs + double(y) * double(y)));
+    }
+    T fi = floor((r - p.Rmin) / p.dR);
+    fi = fi > T(0) ? fi : T(0);
+    if (!(fi <= T(p.nr - 2))) return T(0);
+    const int iR = int(fi);
+    const T I0 = p.I[iR], I1 = p.I[iR + 1];
+    return p.sr_to_px * (I0 + (r - T(iR) * p.dR - p.Rmin) * (I1 - I0) / p.dR);
+}
+
+template <typename T>
+__device__ __noinline__ T central_value(const SweepPrm<T>& p) {
+    const T* I = p.I;
+    const int iIN = p.iIN;
+    const T Rmin = p.Rmin, dR = p.dR;
+    T flux = T(0);
+    for (int i = 1; i < iIN; ++i) flux += (Rmin + dR * T(i)) * I[i];
+    flux = T(double(flux) * 2.0);
+    flux += Rmin * I[0] + (Rmin + T(iIN) * dR) * I[iIN];
+    flux *= dR;
+    const double half = double(p.dxy) / 2.0;
+    if constexpr (std::is_same<T, double>::value) {
+        const double I_interp = (I[iIN + 1] - I[iIN]) / dR * (half - (Rmin + dR * iIN)) + I[iIN];
+        flux += ((Rmin + iIN * dR) * I[iIN] + half * I_interp) * (half - (Rmin + iIN * dR));
+        const double area = half * half - Rmin * Rmin;
+        return p.sr_to_px * flux / area;
+    } else {
+        const float edge = Rmin + dR * float(iIN);
+        const float I_interp = float(double((I[iIN + 1] - I[iIN]) / dR) * (half - double(edge)) + double(I[iIN]));
+        const double term = (double(edge * I[iIN]) + half * double(I_interp)) * (half - double(edge));
+        flux = float(double(flux) + term);
+        const float area = float(half * half - double(Rmin) * double(Rmin));
+        return p.sr_to_px * flux / area;
+    }
+}
+
+// Value of quadrant-shifted image pixel (row k, column n) built from a profile.
+template <typename T>
+__device__ __forceinline__ T sweep_shifted(int k, int n, int N, const SweepPrm<T>& p) {
+    if (k == 0 && n == 0) return central_value(p);
+    if (p.rmax == 0) return T(0);
+    const int xi = n < N / 2 ? -n : N - n;
+    const int yi = k < N / 2 ? -k : N - k;
+    return sweep_pixel(xi, yi, p);
+}
+
+// ---------------------------------------------------------------------------
+// row pass: 2 real rows per workgroup packed as one complex N-point FFT
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RowArgs {
+    SweepPrm<T> sw;              // SRC_SWEEP
+    const T* img;                // SRC_IMAGE_*: N rows of ldimg reals (image in the first N)
+    int ldimg;
+    int N, Rs, ldT;
+    const cpx<T>* tw;            // W_N^e table
+    cpx<T>* Tm;                  // out: [2*Rs][ldT]
+};
+
+__device__ __forceinline__ int compact_to_row(int kk, int Rs, int N) { return kk < Rs ? kk : kk + N - 2 * Rs; }
+
+template <typename T, int SRC>
+__device__ __forceinline__ T row_source(const RowArgs<T>& a, int k, int n) {
+    const int N = a.N;
+    if constexpr (SRC == SRC_SWEEP) {
+        return sweep_shifted(k, n, N, a.sw);
+    } else if constexpr (SRC == SRC_IMAGE_SHIFT) {
+        const int i = (k + N / 2) & (N - 1), j = (n + N / 2) & (N - 1);
+        return a.img[size_t(i) * a.ldimg + j];
+    } else {
+        return a.img[size_t(k) * a.ldimg + n];
+    }
+}
+
+template <typename T, int LOG2N, int SRC>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass(RowArgs<T> a) {
+    using S = FftShape<LOG2N>;
+    using F = BlockFft<T, LOG2N, false>;
+    constexpr int N = S::N, NT = S::NT, E = S::E;
+    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    const int tid = threadIdx.x;
+    const int kk_a = 2 * blockIdx.x, kk_b = kk_a + 1;
+    const int ka = compact_to_row(kk_a, a.Rs, N), kb = compact_to_row(kk_b, a.Rs, N);
+    cpx<T> v[E];
+    F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
+        return cpx<T>{row_source<T, SRC>(a, ka, n), row_source<T, SRC>(a, kb, n)};
+    });
+#pragma unroll
+    for (int r = 0; r < E; ++r) lds[lpad(tid + r * NT)] = v[r];
+    __syncthreads();
+    const int C = N / 2 + 1;
+    cpx<T>* outa = a.Tm + size_t(kk_a) * a.ldT;
+    cpx<T>* outb = a.Tm + size_t(kk_b) * a.ldT;
+    for (int l = tid; l < C; l += NT) {
+        const cpx<T> za = lds[lpad(l)];
+        const cpx<T> zb = lds[lpad((N - l) & (N - 1))];
+        outa[l] = cpx<T>{T(0.5) * (za.re + zb.re), T(0.5) * (za.im - zb.im)};
+        outb[l] = cpx<T>{T(0.5) * (za.im + zb.im), T(0.5) * (zb.re - za.re)};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// column pass: one column of the row-transformed image per workgroup
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ColArgs {
+    const cpx<T>* Tm;
+    int N, Rs, ldT;
+    const cpx<T>* tw;
+    cpx<T>* S;
+    size_t ldS;      // OUT_COLMAJOR_SHIFT: column stride; OUT_ROWMAJOR_PLAIN: unused (row stride C)
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous run of columns so that the strided reads of Tm share L2 lines.
+__device__ __forceinline__ int xcd_column(int b, int nblocks) {
+    const int per = nblocks >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <typename T, int LOG2N, int OUT>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass(ColArgs<T> a) {
+    using S = FftShape<LOG2N>;
+    using F = BlockFft<T, LOG2N, false>;
+    constexpr int N = S::N, NT = S::NT, E = S::E;
+    constexpr int C = N / 2 + 1;
+    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    const int col = xcd_column(blockIdx.x, gridDim.x);
+    if (col >= C) return;
+    const int tid = threadIdx.x;
+    const int Rs = a.Rs;
+    const cpx<T>* src = a.Tm + col;
+    const size_t ldT = a.ldT;
+    cpx<T> v[E];
+    F::run(v, lds, tid, Rs, a.tw, [&](int n) -> cpx<T> {
+        const int kk = n < Rs ? n : n - N + 2 * Rs;
+        return src[size_t(kk) * ldT];
+    });
+    if constexpr (OUT == OUT_COLMAJOR_SHIFT) {
+        cpx<T>* dst = a.S + size_t(col) * a.ldS;
+#pragma unroll
+        for (int r = 0; r < E; ++r) dst[(tid + r * NT + N / 2) & (N - 1)] = v[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; ++r) a.S[size_t(tid + r * NT) * C + col] = v[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// generic even N (not a power of two, or too small/large for the LDS FFT):
+// direct DFT with exact twiddle indices, O(N) per output.
+// ---------------------------------------------------------------------------
+template <typename T, int SRC>
+__global__ __launch_bounds__(256) void k_rowdft_generic(RowArgs<T> a) {
+    extern __shared__ unsigned char smem_raw[];
+    T* x = reinterpret_cast<T*>(smem_raw);           // 2*Rs support values
+    const int N = a.N, Rs = a.Rs, C = N / 2 + 1;
+    const int kk = blockIdx.x;
+    const int k = compact_to_row(kk, Rs, N);
+    for (int s = threadIdx.x; s < 2 * Rs; s += blockDim.x) {
+        const int n = s < Rs ? s : s + N - 2 * Rs;
+        x[s] = row_source<T, SRC>(a, k, n);
+    }
+    __syncthreads();
+    cpx<T>* out = a.Tm + size_t(kk) * a.ldT;
+    for (int l = threadIdx.x; l < C; l += blockDim.x) {
+        T re = 0, im = 0;
+        int e = 0;                                     // (n*l) mod N for n = 0, 1, ...
+        for (int s = 0; s < Rs; ++s) {
+            const cpx<T> w = a.tw[e];
+            re = ffma(x[s], w.re, re);
+            im = ffma(x[s], w.im, im);
+            e += l; if (e >= N) e -= N;
+        }
+        e = int((long long)(N - Rs) * l % N);
+        for (int s = Rs; s < 2 * Rs; ++s) {
+            const cpx<T> w = a.tw[e];
+            re = ffma(x[s], w.re, re);
+            im = ffma(x[s], w.im, im);
+            e += l; if (e >= N) e -= N;
+        }
+        out[l] = cpx<T>{re, im};
+    }
+}
+
+template <typename T, int OUT>
+__global__ __launch_bounds__(256) void k_coldft_generic(ColArgs<T> a) {
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* x = reinterpret_cast<cpx<T>*>(smem_raw);  // 2*Rs support values
+    const int N = a.N, Rs = a.Rs, C = N / 2 + 1;
+    const int col = blockIdx.x;
+    for (int s = threadIdx.x; s < 2 * Rs; s += blockDim.x) x[s] = a.Tm[size_t(s) * a.ldT + col];
+    __syncthreads();
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        T re = 0, im = 0;
+        int e = 0;
+        for (int s = 0; s < Rs; ++s) {
+            const cpx<T> w = a.tw[e];
+            re = ffma(x[s].re, w.re, ffma(-x[s].im, w.im, re));
+            im = ffma(x[s].re, w.im, ffma(x[s].im, w.re, im));
+            e += k; if (e >= N) e -= N;
+        }
+        e = int((long long)(N - Rs) * k % N);
+        for (int s = Rs; s < 2 * Rs; ++s) {
+            const cpx<T> w = a.tw[e];
+            re = ffma(x[s].re, w.re, ffma(-x[s].im, w.im, re));
+            im = ffma(x[s].re, w.im, ffma(x[s].im, w.re, im));
+            e += k; if (e >= N) e -= N;
+        }
+        if constexpr (OUT == OUT_COLMAJOR_SHIFT) {
+            int r = k + N / 2; if (r >= N) r -= N;
+            a.S[size_t(col) * a.ldS + r] = cpx<T>{re, im};
+        } else {
+            a.S[size_t(k) * C + col] = cpx<T>{re, im};
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// rotate -> interpolate -> phase -> (visibilities | chi2 partial sums)
+// interpolate_core galario.cpp:749-815, apply_phase_sampled_core 877-884,
+// uv_rotate_core 955-960, diff_weighted_core 1453-1461.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SampleArgs {
+    const cpx<T>* S;
+    long long rs, cs;            // element (row, col) at row*rs + col*cs
+    int nrow, ncol;
+    T v_origin, duv;
+    int rotate;  T cosPA, sinPA;
+    int phase;   T dRA2, dDec2;  // 2*pi*dRArot, 2*pi*dDecrot
+    int nd;
+    const T* u; const T* v;
+    cpx<T>* vis;                 // MODE_VIS
+    const T* re; const T* im; const T* w;
+    double* partials;            // MODE_CHI2: one per workgroup
+    unsigned* flag;              // bit 0: a point fell outside the grid
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> sample_point(const SampleArgs<T>& a, T u0, T v0, bool& oob) {
+    T urot = u0, vrot = v0;
+    if (a.rotate) {
+        urot = u0 * a.cosPA - v0 * a.sinPA;
+        vrot = u0 * a.sinPA + v0 * a.cosPA;
+    }
+    const T indu = fabs(urot) / a.duv;
+    const T sign_u = copysign(T(1), urot);
+    const T indv = T(a.nrow / 2) + a.v_origin * sign_u * vrot / a.duv;
+    // inside the grid, or on its last row/column with a zero weight beyond it
+    oob = !(indu <= T(a.ncol - 1) && indv >= T(0) && indv <= T(a.nrow - 1));
+    int fl_u = oob ? 0 : int(floor(indu));
+    int fl_v = oob ? 0 : int(floor(indv));
+    const T q = indu - T(fl_u);
+    const T t = indv - T(fl_v);
+    const int fu1 = fl_u + 1 < a.ncol ? fl_u + 1 : fl_u;     // weight q == 0 there
+    const int fv1 = fl_v + 1 < a.nrow ? fl_v + 1 : fl_v;     // weight t == 0 there
+    const cpx<T> y0 = a.S[fl_v * a.rs + fl_u * a.cs];
+    const cpx<T> y1 = a.S[fv1 * a.rs + fl_u * a.cs];
+    const cpx<T> y2 = a.S[fv1 * a.rs + fu1 * a.cs];
+    const cpx<T> y3 = a.S[fl_v * a.rs + fu1 * a.cs];
+    // Z = t*(q*((y0+y2)-(y1+y3))) + (t*(y1-y0) + q*(y3-y0)) + y0
+    const T df_re = (y0.re + y2.re) - (y1.re + y3.re), df_im = (y0.im + y2.im) - (y1.im + y3.im);
+    const T t1_re = t * (q * df_re), t1_im = t * (q * df_im);
+    const T t2_re = t * (y1.re - y0.re), t2_im = t * (y1.im - y0.im);
+    const T t3_re = q * (y3.re - y0.re), t3_im = q * (y3.im - y0.im);
+    const T z_re = (t1_re + (t2_re + t3_re)) + y0.re;
+    const T z_im = (t1_im + (t2_im + t3_im)) + y0.im;
+    const T ph = atan2(z_im, z_re) * sign_u;
+    const T a0 = hypot(y0.re, y0.im), a1 = hypot(y1.re, y1.im);
+    const T a2 = hypot(y2.re, y2.im), a3 = hypot(y3.re, y3.im);
+    T amp = a0;
+    amp += (a3 - a0) * q;
+    amp += (a1 - a0) * t;
+    amp += (((a0 - a1) + a2) - a3) * t * q;
+    T sp, cp;
+    fsincos(ph, &sp, &cp);
+    cpx<T> V = {amp * cp, amp * sp};
+    if (a.phase) {
+        const T ang = urot * a.dRA2 + vrot * a.dDec2;
+        T s, c;
+        fsincos(ang, &s, &c);
+        V = cpx<T>{V.re * c - V.im * s, V.re * s + V.im * c};
+    }
+    if (oob) V = cpx<T>{T(NAN), T(NAN)};
+    return V;
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_sample(SampleArgs<T> a) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    bool oob = false;
+    if (i < a.nd) {
+        const cpx<T> V = sample_point(a, a.u[i], a.v[i], oob);
+        if constexpr (MODE == MODE_VIS) {
+            a.vis[i] = V;
+        } else {
+            const T sw = sqrt(a.w[i]);
+            const T cr = (V.re - a.re[i]) * sw;
+            const T ci = (V.im - a.im[i]) * sw;
+            acc = double(cr * cr + ci * ci);
+            if constexpr (!std::is_same<T, double>::value) acc = double(cr) * cr + double(ci) * ci;
+        }
+    }
+    if (oob && a.flag) atomicOr(a.flag, 1u);
+    if constexpr (MODE == MODE_CHI2) {
+        acc = wave_sum(acc);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    }
+}
+
+// Deterministic final sum of n partials (one workgroup of 1024 threads).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict__ p, int n, double* out_d, T* out_t) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) acc += p[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < 16; ++k) s += red[k];
+        if (out_d) out_d[0] = s;
+        if (out_t) out_t[0] = T(s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// expert stages
+// ---------------------------------------------------------------------------
+// Full nxy x nxy image of a profile (create_image_h without the padding columns).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_image(SweepPrm<T> p, int N, int ld, T* img) {
+    const size_t idx = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= size_t(N) * ld) return;
+    const int i = int(idx / ld), j = int(idx % ld);
+    T val = T(0);
+    const int lo = N / 2 - p.rmax, hi = N / 2 + p.rmax;
+    if (i == N / 2 && j == N / 2) {
+        val = central_value(p);
+    } else if (i >= lo && i < hi && j >= lo && j < hi) {
+        val = sweep_pixel(N / 2 - j, N / 2 - i, p);
+    }
+    img[idx] = val;
+}
+
+// uv_rotate_core (galario.cpp:955-960) over arrays.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rotate(int nd, T c, T s, const T* u, const T* v, T* ur, T* vr) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nd) return;
+    const T u0 = u[i], v0 = v[i];
+    ur[i] = u0 * c - v0 * s;
+    vr[i] = u0 * s + v0 * c;
+}
+
+// apply_phase_sampled_core (galario.cpp:877-884), in place; dRA2/dDec2 include 2*pi.
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_phase(int nd, T dRA2, T dDec2, const T* u, const T* v, cpx<T>* vis) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nd) return;
+    const T ang = u[i] * dRA2 + v[i] * dDec2;
+    T s, c;
+    fsincos(ang, &s, &c);
+    const cpx<T> V = vis[i];
+    vis[i] = cpx<T>{V.re * c - V.im * s, V.re * s + V.im * c};
+}
+
+// chi2 terms of given visibilities (reduce_chi2, galario.cpp:1506-1538).
+template <typename T>
+__global__ __launch_bounds__(256) void k_chi2_terms(int nd, const T* re, const T* im, const cpx<T>* vis, const T* w,
+                                                    double* partials) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (i < nd) {
+        const T sw = sqrt(w[i]);
+        const cpx<T> V = vis[i];
+        const T cr = (V.re - re[i]) * sw;
+        const T ci = (V.im - im[i]) * sw;
+        acc = std::is_same<T, double>::value ? double(cr * cr + ci * ci) : double(cr) * cr + double(ci) * ci;
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Quadrant swap of the N x N real image held in rows of ld reals
+// (shift_core, galario.cpp:565-589); the padding columns are copied through.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fftshift_padded(int N, int ld, const T* in, T* out) {
+    const size_t idx = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= size_t(N) * ld) return;
+    const int i = int(idx / ld), j = int(idx % ld);
+    out[idx] = j < N ? in[size_t((i + N / 2) % N) * ld + (j + N / 2) % N] : in[idx];
+}
+
+// Swap of the upper and lower row halves of an nrow x ncol complex matrix
+// (shift_axis0_core, galario.cpp:657-668).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fftshift_axis0(int nrow, int ncol, cpx<T>* m) {
+    const size_t idx = size_t(blockIdx.x) * 256 + threadIdx.x;
+    const size_t half = size_t(nrow / 2) * ncol;
+    if (idx >= half) return;
+    const cpx<T> a = m[idx];
+    m[idx] = m[idx + half];
+    m[idx + half] = a;
+}
+
+}  // namespace gl

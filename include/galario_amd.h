@@ -1,0 +1,206 @@
+/*
+ * galario_amd -- C ABI of the MI355X (gfx950) HIP implementation of galario's
+ * model-to-visibility path.
+ *
+ * Conventions
+ *  - Every entry point returns a status: GALARIO_OK (0) or one of the error
+ *    codes below; galario_last_error() gives the calling thread's message.
+ *    The codes map onto the C++ exceptions galario's core throws
+ *    (src/galario.cpp:89-121,158-198; docs/py-api.rst:72-82):
+ *        GALARIO_ERR_INVALID  <- std::invalid_argument  (Python ValueError)
+ *        GALARIO_ERR_ALLOC    <- std::bad_alloc         (Python MemoryError)
+ *        GALARIO_ERR_RUNTIME  <- std::runtime_error     (Python RuntimeError)
+ *  - Suffix _f64 computes in double (galario "double"), _f32 in float
+ *    (galario "single").  Complex arrays are interleaved (re, im) pairs, the
+ *    layout of std::complex / numpy complex.
+ *  - Host-pointer entry points (no _dev suffix) copy their inputs to the
+ *    current GPU, compute there and copy the results back before returning;
+ *    no host pointer is retained.  They are thread-safe: each host thread
+ *    uses its own stream and device workspace.
+ *  - _dev entry points take device pointers and a hipStream_t (as void*);
+ *    they enqueue work and return without synchronising.  Calls from one
+ *    host thread share that thread's workspace and are ordered across
+ *    streams by the library.
+ *  - check_central: 1 keeps galario's central-pixel guard
+ *    (CHECK_CENTRAL_PIXEL, src/galario.cpp:113-121: (dxy/2-Rmin)/dR >= 5);
+ *    0 skips it (the arithmetic below the guard stays defined whenever
+ *    0 <= floor((dxy/2-Rmin)/dR) and floor(...)+1 < nr, which is checked).
+ */
+#ifndef GALARIO_AMD_H
+#define GALARIO_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GALARIO_OK 0
+#define GALARIO_ERR_INVALID 1
+#define GALARIO_ERR_ALLOC 2
+#define GALARIO_ERR_RUNTIME 3
+
+/* Message of the last failed call on this thread (empty if none). */
+const char *galario_last_error(void);
+
+/* ---- management: src/galario.h:51-58 ---------------------------------- */
+
+/* replaces galario::init() (src/galario.h:52; python/__init_module__.py.in:22) */
+int galario_init(void);
+/* replaces galario::cleanup() (src/galario.h:53): frees the calling thread's
+ * workspaces and the twiddle tables */
+int galario_cleanup(void);
+/* replaces galario::threads(int) (src/galario.h:54): returns the setting
+ * (default 16, kept for API compatibility; the kernels' launch shapes are fixed
+ * per kernel) */
+int galario_threads(int num);
+/* replaces galario::ngpus() (src/galario.h:57) */
+int galario_ngpus(int *out);
+/* replaces galario::use_gpu(int) (src/galario.h:58): selects the device of
+ * the calling thread */
+int galario_use_gpu(int device_id);
+/* replaces galario::galario_free(void*) (src/galario.h:42) for buffers from
+ * galario_copy_input_* */
+void galario_free(void *ptr);
+
+/* ---- main functions: src/galario_py.h:30-37 -------------------------- */
+
+/* replaces galario::_sample_profile (src/galario_py.h:31) */
+int galario_sample_profile_f64(int nr, const double *intensity, double Rmin, double dR, double dxy, int nxy,
+                               double inc, double dRA, double dDec, double duv, double PA, int nd,
+                               const double *u, const double *v, double *vis, int check_central);
+int galario_sample_profile_f32(int nr, const float *intensity, float Rmin, float dR, float dxy, int nxy,
+                               float inc, float dRA, float dDec, float duv, float PA, int nd,
+                               const float *u, const float *v, float *vis, int check_central);
+
+/* replaces galario::_sample_image (src/galario_py.h:33); image is nx x ny row-major */
+int galario_sample_image_f64(int nx, int ny, const double *image, double v_origin, double dRA, double dDec,
+                             double duv, double PA, int nd, const double *u, const double *v, double *vis);
+int galario_sample_image_f32(int nx, int ny, const float *image, float v_origin, float dRA, float dDec,
+                             float duv, float PA, int nd, const float *u, const float *v, float *vis);
+
+/* replaces galario::_chi2_profile (src/galario_py.h:34) */
+int galario_chi2_profile_f64(int nr, const double *intensity, double Rmin, double dR, double dxy, int nxy,
+                             double inc, double dRA, double dDec, double duv, double PA, int nd,
+                             const double *u, const double *v, const double *vis_obs_re,
+                             const double *vis_obs_im, const double *weights, int check_central,
+                             double *chi2);
+int galario_chi2_profile_f32(int nr, const float *intensity, float Rmin, float dR, float dxy, int nxy,
+                             float inc, float dRA, float dDec, float duv, float PA, int nd,
+                             const float *u, const float *v, const float *vis_obs_re,
+                             const float *vis_obs_im, const float *weights, int check_central,
+                             float *chi2);
+
+/* replaces galario::_chi2_image (src/galario_py.h:36) */
+int galario_chi2_image_f64(int nx, int ny, const double *image, double v_origin, double dRA, double dDec,
+                           double duv, double PA, int nd, const double *u, const double *v,
+                           const double *vis_obs_re, const double *vis_obs_im, const double *weights,
+                           double *chi2);
+int galario_chi2_image_f32(int nx, int ny, const float *image, float v_origin, float dRA, float dDec,
+                           float duv, float PA, int nd, const float *u, const float *v,
+                           const float *vis_obs_re, const float *vis_obs_im, const float *weights,
+                           float *chi2);
+
+/* replaces galario::_sweep (src/galario_py.h:37); image: nxy x (nxy/2+1)
+ * complex = nxy rows of 2*(nxy/2+1) reals, the image in the first nxy reals
+ * of each row (padding set to zero) */
+int galario_sweep_f64(int nr, const double *intensity, double Rmin, double dR, int nxy, double dxy, double inc,
+                      double *image, int check_central);
+int galario_sweep_f32(int nr, const float *intensity, float Rmin, float dR, int nxy, float dxy, float inc,
+                      float *image, int check_central);
+
+/* replaces galario::_uv_rotate (src/galario_py.h:38) */
+int galario_uv_rotate_f64(double PA, double dRA, double dDec, double *dRArot, double *dDecrot, int nd,
+                          const double *u, const double *v, double *urot, double *vrot);
+int galario_uv_rotate_f32(float PA, float dRA, float dDec, float *dRArot, float *dDecrot, int nd,
+                          const float *u, const float *v, float *urot, float *vrot);
+
+/* ---- expert functions: src/galario_py.h:40-46 ------------------------- */
+
+/* replaces galario::_copy_input (src/galario_py.h:40): returns in *out a
+ * host buffer of nx x (ny/2+1) complex holding the rows of realdata padded to
+ * 2*(ny/2+1) reals; free it with galario_free */
+int galario_copy_input_f64(int nx, int ny, const double *realdata, void **out);
+int galario_copy_input_f32(int nx, int ny, const float *realdata, void **out);
+
+/* replaces galario::_fft2d (src/galario_py.h:41): in-place forward R2C 2-D
+ * FFT of the padded buffer (unnormalised, e^{-2 pi i kn/N}) */
+int galario_fft2d_f64(int nx, int ny, void *data);
+int galario_fft2d_f32(int nx, int ny, void *data);
+
+/* replaces galario::_fftshift (src/galario_py.h:42): in-place quadrant swap
+ * of the real image held in the padded buffer */
+int galario_fftshift_f64(int nx, int ny, void *data);
+int galario_fftshift_f32(int nx, int ny, void *data);
+
+/* replaces galario::_fftshift_axis0 (src/galario_py.h:43): in-place swap of
+ * the row halves of an nrow x ncol complex matrix */
+int galario_fftshift_axis0_f64(int nrow, int ncol, void *data);
+int galario_fftshift_axis0_f32(int nrow, int ncol, void *data);
+
+/* replaces galario::_interpolate (src/galario_py.h:44); data is an
+ * nrow x ncol row-major complex matrix (the shifted R2C spectrum).  Points
+ * outside the grid give GALARIO_ERR_INVALID (galario does not check them). */
+int galario_interpolate_f64(int nrow, int ncol, const void *data, double v_origin, int nd, const double *u,
+                            const double *v, double duv, void *vis);
+int galario_interpolate_f32(int nrow, int ncol, const void *data, float v_origin, int nd, const float *u,
+                            const float *v, float duv, void *vis);
+
+/* replaces galario::_apply_phase_sampled (src/galario_py.h:45): vis in place */
+int galario_apply_phase_sampled_f64(double dRA, double dDec, int nd, const double *u, const double *v,
+                                    void *vis);
+int galario_apply_phase_sampled_f32(float dRA, float dDec, int nd, const float *u, const float *v,
+                                    void *vis);
+
+/* replaces galario::_reduce_chi2 (src/galario_py.h:46) */
+int galario_reduce_chi2_f64(int nd, const double *vis_obs_re, const double *vis_obs_im, const void *vis,
+                            const double *weights, double *chi2);
+int galario_reduce_chi2_f32(int nd, const float *vis_obs_re, const float *vis_obs_im, const void *vis,
+                            const float *weights, float *chi2);
+
+/* ---- device-resident forms (no counterpart in galario: the hot path with
+ *      the data already in HBM, for torch callers and the benchmark) ------ */
+
+/* galario::chi2_profile (src/galario.cpp:1615) on device arrays; *chi2_d
+ * (device, double) receives chi2.  Enqueued on `stream`, not synchronised.
+ * Points outside the grid make chi2 NaN and set the flag read by
+ * galario_take_oob_flag. */
+int galario_chi2_profile_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
+                                 double dxy, int nxy, double inc, double dRA, double dDec, double duv,
+                                 double PA, int nd, const double *u, const double *v, const double *vis_obs_re,
+                                 const double *vis_obs_im, const double *weights, int check_central,
+                                 double *chi2_d);
+int galario_chi2_profile_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
+                                 float dxy, int nxy, float inc, float dRA, float dDec, float duv,
+                                 float PA, int nd, const float *u, const float *v, const float *vis_obs_re,
+                                 const float *vis_obs_im, const float *weights, int check_central,
+                                 double *chi2_d);
+
+/* galario::sample_profile (src/galario.cpp:1404) on device arrays; vis_d: nd complex */
+int galario_sample_profile_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
+                                   double dxy, int nxy, double inc, double dRA, double dDec, double duv,
+                                   double PA, int nd, const double *u, const double *v, int check_central,
+                                   void *vis_d);
+int galario_sample_profile_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
+                                   float dxy, int nxy, float inc, float dRA, float dDec, float duv,
+                                   float PA, int nd, const float *u, const float *v, int check_central,
+                                   void *vis_d);
+
+/* Per-stage timing of the calling thread's _dev calls (no counterpart in
+ * galario, whose GALARIO_TIMING build prints cudaEvent timings per stage,
+ * src/galario.cpp:275-315): enable=1 starts recording a HIP event before each
+ * stage (0 row pass, 1 column pass, 2 sample, 3 final sum) and after the last,
+ * on the call's stream; enable=0 stops.  Either clears what was recorded. */
+int galario_stage_timer(int enable);
+/* Waits for the recorded events and writes the summed milliseconds of each
+ * stage to ms[0..nstage-1] and the number of timed calls to *ncalls. */
+int galario_stage_times(double *ms, int nstage, int *ncalls);
+
+/* Synchronises the calling thread's work on the current device and returns
+ * (and clears) its out-of-grid flag: 1 if any point of a _dev call since the
+ * last take fell outside the Fourier grid. */
+int galario_take_oob_flag(int *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GALARIO_AMD_H */
