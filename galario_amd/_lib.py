@@ -7,6 +7,7 @@ build command.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -19,7 +20,26 @@ _vp = ctypes.c_void_p
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
+def _preload_torch_hip_runtime() -> None:
+    """Use the HIP runtime that ships with PyTorch when PyTorch is installed.
+
+    PyTorch-ROCm wheels carry their own libamdhip64.so (soname
+    libamdhip64.so.7, requested by torch as libamdhip64.so).  If this library
+    were loaded first it would bind /opt/rocm's copy, torch would later load
+    its own, and the process would hold two HIP runtimes (torch then reports
+    no GPU).  Loading torch's copy first, globally, makes both share it."""
+    spec = importlib.util.find_spec('torch')
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for base in spec.submodule_search_locations:
+        path = os.path.join(base, 'lib', 'libamdhip64.so')
+        if os.path.exists(path):
+            ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+            return
+
+
 def _load():
+    _preload_torch_hip_runtime()
     if not os.path.exists(LIB_PATH):
         raise ImportError(f'galario_amd: {LIB_PATH} is missing; build it with '
                           f'`python galario_amd/build.py` (hipcc, gfx950)')
@@ -57,7 +77,7 @@ def _load():
             'chi2_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp, _vp,
                                  _vp, _c_int, _vp],
             'sample_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _c_int,
-                                   _vp],
+                                   _vp, _vp],
         }
         for name, args in sig.items():
             f = getattr(lib, f'galario_{name}_{suf}')

@@ -135,7 +135,7 @@ thread_local std::map<int, Workspace*> t_ws;
 // When enabled, the _dev entry points record a HIP event before each stage and
 // after the last one, on the stream the kernels run on; galario_stage_times
 // synchronises and returns the summed durations per stage.
-enum { ST_ROW = 0, ST_COL = 1, ST_SAMPLE = 2, ST_SUM = 3, NSTAGE = 4 };
+enum { ST_ROW = 0, ST_COL = 1, ST_SAMPLE = 2, NSTAGE = 3 };
 struct StageTimer {
     bool on = false;
     std::vector<hipEvent_t> pool;            // free events
@@ -255,6 +255,35 @@ void launch_colpass_L(int blocks, hipStream_t s, const ColArgs<T>& a) {
     hipLaunchKernelGGL((k_colpass<T, L, OUT>), dim3(blocks), dim3(FftShape<L>::NT), 0, s, a);
 }
 
+#define GL_L_SWITCH(T, CALL)                                                   \
+    switch (L) {                                                                \
+        case 7: CALL(7); break;                                                 \
+        case 8: CALL(8); break;                                                 \
+        case 9: CALL(9); break;                                                 \
+        case 10: CALL(10); break;                                               \
+        case 11: CALL(11); break;                                               \
+        case 12: CALL(12); break;                                               \
+        case 13: CALL(13); break;                                               \
+        case 14:                                                                \
+            if constexpr (sizeof(T) == 4) { CALL(14); break; }                  \
+            [[fallthrough]];                                                    \
+        default: fail(GALARIO_ERR_RUNTIME, "internal: no FFT kernel for this size"); \
+    }
+
+template <typename T>
+void launch_rowpass_sym(int L, int blocks, hipStream_t s, const SymRowArgs<T>& a) {
+#define GL_CALL(LL) hipLaunchKernelGGL((k_rowpass_sym<T, LL>), dim3(blocks), dim3(FftShape<LL>::NT), 0, s, a)
+    GL_L_SWITCH(T, GL_CALL)
+#undef GL_CALL
+}
+
+template <typename T>
+void launch_colpass_sym(int L, int blocks, hipStream_t s, const SymColArgs<T>& a) {
+#define GL_CALL(LL) hipLaunchKernelGGL((k_colpass_sym<T, LL>), dim3(blocks), dim3(FftShape<LL>::NT), 0, s, a)
+    GL_L_SWITCH(T, GL_CALL)
+#undef GL_CALL
+}
+
 template <typename T, int SRC>
 void launch_rowpass(int L, int pairs, hipStream_t s, const RowArgs<T>& a) {
     switch (L) {
@@ -330,6 +359,36 @@ cpx<T>* build_spectrum(Workspace& w, hipStream_t s, RowArgs<T> ra, int N, int Rs
     }
     HC(hipGetLastError());
     return S_out;
+}
+
+// Real, even half spectrum H of a swept profile (see k_rowpass_sym); returns
+// false when the workgroup FFT does not cover N (the caller then uses the
+// general complex path).
+template <typename T>
+bool build_spectrum_sym(Workspace& w, hipStream_t s, const SweepPrm<T>& sw, int N, int rmax, const T** H_out,
+                        int* ldH_out) {
+    const int L = fast_log2<T>(N);
+    if (L < 0) return false;
+    const int C = N / 2 + 1;
+    const int Rs = rmax + 1 < N / 2 ? rmax + 1 : N / 2;      // symmetric support half-width
+    const int nrows = Rs + 1;
+    const int pairs_r = (nrows + 1) / 2;
+    const int ldT = (2 * pairs_r + 1) & ~1;                  // column stride of Ts (rows m, padded even)
+    const int ldH = (N / 2 + 1 + 7) & ~7;
+    T* Ts = w.Tm.get<T>(size_t(C + 1) * ldT);                // C+1 columns: the last pair reads one past C
+    T* H = w.S.get<T>(size_t(C + 1) * ldH);
+    const cpx<T>* tw = twiddles<T>(N);
+    SymRowArgs<T> ra{sw, N, Rs, nrows, ldT, tw, Ts};
+    SymColArgs<T> ca{Ts, N, Rs, ldT, tw, H, ldH};
+    const int pairs_c = (C + 1) / 2;
+    t_timer.mark(ST_ROW, s);
+    launch_rowpass_sym<T>(L, pairs_r, s, ra);
+    t_timer.mark(ST_COL, s);
+    launch_colpass_sym<T>(L, 8 * ((pairs_c + 7) / 8), s, ca);
+    HC(hipGetLastError());
+    *H_out = H;
+    *ldH_out = ldH;
+    return true;
 }
 
 // ------------------------------------------------------------------ setup of a profile evaluation
@@ -412,6 +471,8 @@ SampleArgs<T> sample_args(const cpx<T>* S, long long rs, long long cs, int nrow,
                           const SampleSetup<T>& ss, int nd, const T* u, const T* v) {
     SampleArgs<T> a{};
     a.S = S;
+    a.H = nullptr;
+    a.ldH = 0;
     a.rs = rs;
     a.cs = cs;
     a.nrow = nrow;
@@ -435,23 +496,47 @@ template <typename T>
 void launch_chi2(Workspace& w, hipStream_t s, SampleArgs<T> a, const T* re, const T* im, const T* wt,
                  double* chi2_d, T* chi2_t) {
     const unsigned nb = blocks_for(size_t(a.nd), 256);
-    double* part = w.part.get<double>(nb ? nb : 1);
     a.re = re;
     a.im = im;
     a.w = wt;
-    a.partials = part;
+    a.partials = w.part.get<double>(nb ? nb : 1);
     t_timer.mark(ST_SAMPLE, s);
-    if (nb) hipLaunchKernelGGL((k_sample<T, MODE_CHI2>), dim3(nb), dim3(256), 0, s, a);
-    t_timer.mark(ST_SUM, s);
-    hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, part, int(nb), chi2_d, chi2_t);
+    if (nb && a.H) hipLaunchKernelGGL((k_sample<T, MODE_CHI2, true>), dim3(nb), dim3(256), 0, s, a);
+    else if (nb) hipLaunchKernelGGL((k_sample<T, MODE_CHI2, false>), dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, a.partials, int(nb), chi2_d, chi2_t);
     HC(hipGetLastError());
 }
 
 template <typename T>
 void launch_vis(hipStream_t s, SampleArgs<T> a, cpx<T>* vis) {
     a.vis = vis;
-    if (a.nd) hipLaunchKernelGGL((k_sample<T, MODE_VIS>), dim3(blocks_for(size_t(a.nd), 256)), dim3(256), 0, s, a);
+    const unsigned nb = blocks_for(size_t(a.nd), 256);
+    t_timer.mark(ST_SAMPLE, s);
+    if (nb && a.H) hipLaunchKernelGGL((k_sample<T, MODE_VIS, true>), dim3(nb), dim3(256), 0, s, a);
+    else if (nb) hipLaunchKernelGGL((k_sample<T, MODE_VIS, false>), dim3(nb), dim3(256), 0, s, a);
     HC(hipGetLastError());
+}
+
+// Spectrum of a swept profile and the sample arguments that read it: the
+// real, even half spectrum when the workgroup FFT covers nxy, else the
+// general complex spectrum.
+template <typename T>
+SampleArgs<T> profile_spectrum(Workspace& w, hipStream_t s, ProfileSetup<T>& ps, const SampleSetup<T>& ss, int N,
+                               T duv, int nd, const T* u, const T* v) {
+    const int C = N / 2 + 1;
+    const T* H = nullptr;
+    int ldH = 0;
+    if (build_spectrum_sym<T>(w, s, ps.sw, N, ps.rmax, &H, &ldH)) {
+        SampleArgs<T> sa = sample_args<T>(nullptr, 0, 0, N, C, T(1), duv, ss, nd, u, v);
+        sa.H = H;
+        sa.ldH = ldH;
+        return sa;
+    }
+    RowArgs<T> ra{};
+    ra.sw = ps.sw;
+    cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
+    build_spectrum<T, SRC_SWEEP, OUT_COLMAJOR_SHIFT>(w, s, ra, N, ps.rmax > 0 ? ps.rmax : 1, S, size_t(N));
+    return sample_args<T>(S, 1, N, N, C, T(1), duv, ss, nd, u, v);
 }
 
 // ------------------------------------------------------------------ host transfer helpers
@@ -498,12 +583,7 @@ int sample_profile_host(int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc,
         ps.sw.I = upload(w.in[0], I, size_t(nr), s);
         const T* du = upload(w.in[1], u, size_t(nd), s);
         const T* dv = upload(w.in[2], v, size_t(nd), s);
-        const int N = nxy, C = N / 2 + 1;
-        RowArgs<T> ra{};
-        ra.sw = ps.sw;
-        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
-        build_spectrum<T, SRC_SWEEP, OUT_COLMAJOR_SHIFT>(w, s, ra, N, ps.rmax > 0 ? ps.rmax : 1, S, size_t(N));
-        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, T(1), duv, ss, nd, du, dv);
+        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, du, dv);
         sa.flag = flag;
         cpx<T>* dvis = w.out.get<cpx<T>>(size_t(nd));
         launch_vis(s, sa, dvis);
@@ -528,12 +608,7 @@ int chi2_profile_host(int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T
         const T* dre = upload(w.in[3], re, size_t(nd), s);
         const T* dim = upload(w.in[4], im, size_t(nd), s);
         const T* dw = upload(w.in[5], wt, size_t(nd), s);
-        const int N = nxy, C = N / 2 + 1;
-        RowArgs<T> ra{};
-        ra.sw = ps.sw;
-        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
-        build_spectrum<T, SRC_SWEEP, OUT_COLMAJOR_SHIFT>(w, s, ra, N, ps.rmax > 0 ? ps.rmax : 1, S, size_t(N));
-        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, T(1), duv, ss, nd, du, dv);
+        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, du, dv);
         sa.flag = flag;
         double* res = w.scal.get<double>(4);
         launch_chi2<T>(w, s, sa, dre, dim, dw, res, nullptr);
@@ -766,10 +841,11 @@ int reduce_chi2_host(int nd, const T* re, const T* im, const void* vis, const T*
         const T* dw = upload(w.in[5], wt, size_t(nd), s);
         const cpx<T>* dvis = upload(w.out, static_cast<const cpx<T>*>(vis), size_t(nd), s);
         const unsigned nb = blocks_for(size_t(nd), 256);
-        double* part = w.part.get<double>(nb ? nb : 1);
         double* res = w.scal.get<double>(4);
-        if (nb) hipLaunchKernelGGL((k_chi2_terms<T>), dim3(nb), dim3(256), 0, s, nd, dre, dim, dvis, dw, part);
-        hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, part, int(nb), res, (T*)nullptr);
+        SampleArgs<T> a{};
+        a.partials = w.part.get<double>(nb ? nb : 1);
+        if (nb) hipLaunchKernelGGL((k_chi2_terms<T>), dim3(nb), dim3(256), 0, s, nd, dre, dim, dvis, dw, a);
+        hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, a.partials, int(nb), res, (T*)nullptr);
         HC(hipGetLastError());
         download(w.h_scal, res, 1, s);
         HC(hipStreamSynchronize(s));
@@ -789,12 +865,7 @@ int chi2_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int 
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
-        const int N = nxy, C = N / 2 + 1;
-        RowArgs<T> ra{};
-        ra.sw = ps.sw;
-        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
-        build_spectrum<T, SRC_SWEEP, OUT_COLMAJOR_SHIFT>(w, s, ra, N, ps.rmax > 0 ? ps.rmax : 1, S, size_t(N));
-        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, T(1), duv, ss, nd, u, v);
+        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, u, v);
         sa.flag = w.flag_dev.get<unsigned>(1);
         launch_chi2<T>(w, s, sa, re, im, wt, chi2_d, nullptr);
         t_timer.end_call(s);
@@ -803,7 +874,7 @@ int chi2_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int 
 
 template <typename T>
 int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec, T duv,
-                       T PA, int nd, const T* u, const T* v, int check_central, void* vis_d) {
+                       T PA, int nd, const T* u, const T* v, int check_central, const int* perm, void* vis_d) {
     return guard([&] {
         check_nd(nd);
         ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
@@ -811,13 +882,9 @@ int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, in
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
-        const int N = nxy, C = N / 2 + 1;
-        RowArgs<T> ra{};
-        ra.sw = ps.sw;
-        cpx<T>* S = w.S.get<cpx<T>>(size_t(C) * N);
-        build_spectrum<T, SRC_SWEEP, OUT_COLMAJOR_SHIFT>(w, s, ra, N, ps.rmax > 0 ? ps.rmax : 1, S, size_t(N));
-        SampleArgs<T> sa = sample_args<T>(S, 1, N, N, C, T(1), duv, ss, nd, u, v);
+        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, u, v);
         sa.flag = w.flag_dev.get<unsigned>(1);
+        sa.perm = perm;
         launch_vis(s, sa, static_cast<cpx<T>*>(vis_d));
     });
 }
@@ -930,9 +997,9 @@ void galario_free(void* ptr) { std::free(ptr); }
     }                                                                                                             \
     int galario_sample_profile_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc,  \
                                          T dRA, T dDec, T duv, T PA, int nd, const T* u, const T* v, int cc,     \
-                                         void* vis_d) {                                                          \
+                                         const int* perm, void* vis_d) {                                         \
         return sample_profile_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v, cc,   \
-                                     vis_d);                                                                      \
+                                     perm, vis_d);                                                               \
     }
 
 GL_BOTH(GL_MAIN)

@@ -116,7 +116,8 @@ __device__ __forceinline__ T row_source(const RowArgs<T>& a, int k, int n) {
     if constexpr (SRC == SRC_SWEEP) {
         return sweep_shifted(k, n, N, a.sw);
     } else if constexpr (SRC == SRC_IMAGE_SHIFT) {
-        const int i = (k + N / 2) & (N - 1), j = (n + N / 2) & (N - 1);
+        const int i = k + N / 2 < N ? k + N / 2 : k - N / 2;
+        const int j = n + N / 2 < N ? n + N / 2 : n - N / 2;
         return a.img[size_t(i) * a.ldimg + j];
     } else {
         return a.img[size_t(k) * a.ldimg + n];
@@ -268,14 +269,110 @@ __global__ __launch_bounds__(256) void k_coldft_generic(ColArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
-// rotate -> interpolate -> phase -> (visibilities | chi2 partial sums)
+// profile path with the image symmetry: a swept radial profile depends on
+// x^2 and y^2 only, and the quadrant-shifted image z[k][n] is exactly even in
+// both indices (z[N-k][n] = z[k][n] = z[k][N-n]; the box rows/columns that
+// have no mirror inside the box are zero, since there r >= rmax*dxy puts the
+// radial index at nr-1 or beyond).  Its DFT is therefore real and even in both
+// frequencies: the row pass packs rows m_a, m_b as one complex FFT and reads
+// the two real results off the real and imaginary parts; the column pass does
+// the same for two columns and stores frequencies 0..N/2 only.
+//   Ts : real, column-major [C+1][ldT]: Ts[l*ldT + m] = row transform of image
+//        row m = |y| pixel index (m = 0..Rs') at u-frequency l; column C is
+//        zero (partner of column C-1 in the last column pair).
+//   H  : real, column-major [C][ldH]: H[l*ldH + m] = S at u-column l and
+//        |v-frequency| m; S at shifted row r is H[.. + |r - N/2|].
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SymRowArgs {
+    SweepPrm<T> sw;
+    int N, Rs;            // symmetric support half-width Rs' (columns [0,Rs') U [N-Rs',N))
+    int nrows;            // Rs' + 1 rows m = 0..Rs'
+    int ldT;              // column stride of Ts (>= 2 * number of row pairs)
+    const cpx<T>* tw;
+    T* Ts;
+};
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass_sym(SymRowArgs<T> a) {
+    using S = FftShape<LOG2N>;
+    using F = BlockFft<T, LOG2N, false>;
+    constexpr int N = S::N, NT = S::NT, E = S::E;
+    constexpr int C = N / 2 + 1;
+    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    const int tid = threadIdx.x;
+    const int ma = 2 * blockIdx.x, mb = ma + 1;
+    const bool hb = mb < a.nrows;
+    cpx<T> v[E];
+    F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
+        const int xi = n < N / 2 ? -n : N - n;
+        const T za = (ma == 0 && n == 0) ? central_value(a.sw) : sweep_pixel(xi, ma, a.sw);
+        const T zb = hb ? sweep_pixel(xi, mb, a.sw) : T(0);
+        return cpx<T>{za, zb};
+    });
+    // rows ma, mb are adjacent in each column of Ts: one 16-byte store per frequency
+    cpx<T>* out = reinterpret_cast<cpx<T>*>(a.Ts + ma);
+    const size_t ld2 = size_t(a.ldT) / 2;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int k = tid + r * NT;
+        if (r * NT < C && k < C) out[size_t(k) * ld2] = v[r];
+    }
+    if (tid == 0) out[size_t(C) * ld2] = cpx<T>{T(0), T(0)};   // padding column C
+}
+
+template <typename T>
+struct SymColArgs {
+    const T* Ts;
+    int N, Rs, ldT;
+    const cpx<T>* tw;
+    T* H;
+    int ldH;
+};
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass_sym(SymColArgs<T> a) {
+    using S = FftShape<LOG2N>;
+    using F = BlockFft<T, LOG2N, false>;
+    constexpr int N = S::N, NT = S::NT, E = S::E;
+    constexpr int C = N / 2 + 1;
+    constexpr int NPAIR = (C + 1) / 2;
+    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    const int pair = xcd_column(blockIdx.x, gridDim.x);
+    if (pair >= NPAIR) return;
+    const int la = 2 * pair, lb = la + 1;
+    const int tid = threadIdx.x;
+    const T* ca = a.Ts + size_t(la) * a.ldT;     // columns la, lb of Ts: contiguous runs of rows m
+    const T* cb = ca + a.ldT;
+    cpx<T> v[E];
+    F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
+        const int m = n < N / 2 ? n : N - n;
+        return cpx<T>{ca[m], cb[m]};
+    });
+    T* ha = a.H + size_t(la) * a.ldH;
+    T* hb = ha + a.ldH;
+    const bool has_b = lb < C;
+#pragma unroll
+    for (int r = 0; r <= E / 2; ++r) {
+        const int k = tid + r * NT;
+        if (k <= N / 2) {
+            ha[k] = v[r].re;
+            if (has_b) hb[k] = v[r].im;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// rotate -> interpolate -> phase -> (visibilities | chi2)
 // interpolate_core galario.cpp:749-815, apply_phase_sampled_core 877-884,
 // uv_rotate_core 955-960, diff_weighted_core 1453-1461.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct SampleArgs {
-    const cpx<T>* S;
+    const cpx<T>* S;             // complex spectrum (general path)
     long long rs, cs;            // element (row, col) at row*rs + col*cs
+    const T* H;                  // real half spectrum (symmetric path), column stride ldH
+    int ldH;
     int nrow, ncol;
     T v_origin, duv;
     int rotate;  T cosPA, sinPA;
@@ -283,33 +380,63 @@ struct SampleArgs {
     int nd;
     const T* u; const T* v;
     cpx<T>* vis;                 // MODE_VIS
+    const int* perm;             // MODE_VIS: point i is written to vis[perm[i]] (null: vis[i])
     const T* re; const T* im; const T* w;
     double* partials;            // MODE_CHI2: one per workgroup
     unsigned* flag;              // bit 0: a point fell outside the grid
 };
 
+// Grid coordinates of a (rotated) point; oob when it is not inside the grid
+// (or on its last row/column, where the weight beyond is zero).
+template <typename T>
+struct GridPos {
+    int fl_u, fl_v, fu1, fv1;
+    T q, t, sign_u, urot, vrot;
+    bool oob;
+};
+
+template <typename T>
+__device__ __forceinline__ GridPos<T> grid_pos(const SampleArgs<T>& a, T u0, T v0) {
+    GridPos<T> g;
+    g.urot = u0;
+    g.vrot = v0;
+    if (a.rotate) {
+        g.urot = u0 * a.cosPA - v0 * a.sinPA;
+        g.vrot = u0 * a.sinPA + v0 * a.cosPA;
+    }
+    const T indu = fabs(g.urot) / a.duv;
+    g.sign_u = copysign(T(1), g.urot);
+    const T indv = T(a.nrow / 2) + a.v_origin * g.sign_u * g.vrot / a.duv;
+    g.oob = !(indu <= T(a.ncol - 1) && indv >= T(0) && indv <= T(a.nrow - 1));
+    g.fl_u = g.oob ? 0 : int(floor(indu));
+    g.fl_v = g.oob ? 0 : int(floor(indv));
+    g.q = indu - T(g.fl_u);
+    g.t = indv - T(g.fl_v);
+    g.fu1 = g.fl_u + 1 < a.ncol ? g.fl_u + 1 : g.fl_u;   // weight q == 0 there
+    g.fv1 = g.fl_v + 1 < a.nrow ? g.fl_v + 1 : g.fl_v;   // weight t == 0 there
+    return g;
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> apply_phase_pt(const SampleArgs<T>& a, const GridPos<T>& g, cpx<T> V) {
+    if (a.phase) {
+        const T ang = g.urot * a.dRA2 + g.vrot * a.dDec2;
+        T s, c;
+        fsincos(ang, &s, &c);
+        V = cpx<T>{V.re * c - V.im * s, V.re * s + V.im * c};
+    }
+    return V;
+}
+
 template <typename T>
 __device__ __forceinline__ cpx<T> sample_point(const SampleArgs<T>& a, T u0, T v0, bool& oob) {
-    T urot = u0, vrot = v0;
-    if (a.rotate) {
-        urot = u0 * a.cosPA - v0 * a.sinPA;
-        vrot = u0 * a.sinPA + v0 * a.cosPA;
-    }
-    const T indu = fabs(urot) / a.duv;
-    const T sign_u = copysign(T(1), urot);
-    const T indv = T(a.nrow / 2) + a.v_origin * sign_u * vrot / a.duv;
-    // inside the grid, or on its last row/column with a zero weight beyond it
-    oob = !(indu <= T(a.ncol - 1) && indv >= T(0) && indv <= T(a.nrow - 1));
-    int fl_u = oob ? 0 : int(floor(indu));
-    int fl_v = oob ? 0 : int(floor(indv));
-    const T q = indu - T(fl_u);
-    const T t = indv - T(fl_v);
-    const int fu1 = fl_u + 1 < a.ncol ? fl_u + 1 : fl_u;     // weight q == 0 there
-    const int fv1 = fl_v + 1 < a.nrow ? fl_v + 1 : fl_v;     // weight t == 0 there
-    const cpx<T> y0 = a.S[fl_v * a.rs + fl_u * a.cs];
-    const cpx<T> y1 = a.S[fv1 * a.rs + fl_u * a.cs];
-    const cpx<T> y2 = a.S[fv1 * a.rs + fu1 * a.cs];
-    const cpx<T> y3 = a.S[fl_v * a.rs + fu1 * a.cs];
+    const GridPos<T> g = grid_pos(a, u0, v0);
+    oob = g.oob;
+    const T q = g.q, t = g.t;
+    const cpx<T> y0 = a.S[g.fl_v * a.rs + g.fl_u * a.cs];
+    const cpx<T> y1 = a.S[g.fv1 * a.rs + g.fl_u * a.cs];
+    const cpx<T> y2 = a.S[g.fv1 * a.rs + g.fu1 * a.cs];
+    const cpx<T> y3 = a.S[g.fl_v * a.rs + g.fu1 * a.cs];
     // Z = t*(q*((y0+y2)-(y1+y3))) + (t*(y1-y0) + q*(y3-y0)) + y0
     const T df_re = (y0.re + y2.re) - (y1.re + y3.re), df_im = (y0.im + y2.im) - (y1.im + y3.im);
     const T t1_re = t * (q * df_re), t1_im = t * (q * df_im);
@@ -317,7 +444,7 @@ __device__ __forceinline__ cpx<T> sample_point(const SampleArgs<T>& a, T u0, T v
     const T t3_re = q * (y3.re - y0.re), t3_im = q * (y3.im - y0.im);
     const T z_re = (t1_re + (t2_re + t3_re)) + y0.re;
     const T z_im = (t1_im + (t2_im + t3_im)) + y0.im;
-    const T ph = atan2(z_im, z_re) * sign_u;
+    const T ph = atan2(z_im, z_re) * g.sign_u;
     const T a0 = hypot(y0.re, y0.im), a1 = hypot(y1.re, y1.im);
     const T a2 = hypot(y2.re, y2.im), a3 = hypot(y3.re, y3.im);
     T amp = a0;
@@ -326,45 +453,47 @@ __device__ __forceinline__ cpx<T> sample_point(const SampleArgs<T>& a, T u0, T v
     amp += (((a0 - a1) + a2) - a3) * t * q;
     T sp, cp;
     fsincos(ph, &sp, &cp);
-    cpx<T> V = {amp * cp, amp * sp};
-    if (a.phase) {
-        const T ang = urot * a.dRA2 + vrot * a.dDec2;
-        T s, c;
-        fsincos(ang, &s, &c);
-        V = cpx<T>{V.re * c - V.im * s, V.re * s + V.im * c};
-    }
+    cpx<T> V = apply_phase_pt(a, g, cpx<T>{amp * cp, amp * sp});
     if (oob) V = cpx<T>{T(NAN), T(NAN)};
     return V;
 }
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_sample(SampleArgs<T> a) {
-    __shared__ double red[4];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    double acc = 0.0;
-    bool oob = false;
-    if (i < a.nd) {
-        const cpx<T> V = sample_point(a, a.u[i], a.v[i], oob);
-        if constexpr (MODE == MODE_VIS) {
-            a.vis[i] = V;
-        } else {
-            const T sw = sqrt(a.w[i]);
-            const T cr = (V.re - a.re[i]) * sw;
-            const T ci = (V.im - a.im[i]) * sw;
-            acc = double(cr * cr + ci * ci);
-            if constexpr (!std::is_same<T, double>::value) acc = double(cr) * cr + double(ci) * ci;
-        }
-    }
-    if (oob && a.flag) atomicOr(a.flag, 1u);
-    if constexpr (MODE == MODE_CHI2) {
-        acc = wave_sum(acc);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) a.partials[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-    }
+// Same on the real, even spectrum H: the corners are real, Z is real, the
+// phase of galario's amplitude/phase interpolation is 0 or pi, so
+// V = (copysign(A, Z), 0) before the phase shift.
+template <typename T>
+__device__ __forceinline__ cpx<T> sample_point_sym(const SampleArgs<T>& a, T u0, T v0, bool& oob) {
+    const GridPos<T> g = grid_pos(a, u0, v0);
+    oob = g.oob;
+    const T q = g.q, t = g.t;
+    const int half = a.nrow / 2;
+    const int m0 = abs(g.fl_v - half), m1 = abs(g.fv1 - half);
+    const T* c0 = a.H + size_t(g.fl_u) * a.ldH;
+    const T* c1 = a.H + size_t(g.fu1) * a.ldH;
+    const T y0 = c0[m0], y1 = c0[m1], y3 = c1[m0], y2 = c1[m1];
+    const T z = (t * (q * ((y0 + y2) - (y1 + y3))) + (t * (y1 - y0) + q * (y3 - y0))) + y0;
+    const T a0 = fabs(y0), a1 = fabs(y1), a2 = fabs(y2), a3 = fabs(y3);
+    T amp = a0;
+    amp += (a3 - a0) * q;
+    amp += (a1 - a0) * t;
+    amp += (((a0 - a1) + a2) - a3) * t * q;
+    cpx<T> V = apply_phase_pt(a, g, cpx<T>{copysign(amp, z), T(0)});
+    if (oob) V = cpx<T>{T(NAN), T(NAN)};
+    return V;
 }
 
-// Deterministic final sum of n partials (one workgroup of 1024 threads).
+// Deterministic block sum of `acc` (blockDim 256) into *out on thread 0.
+__device__ __forceinline__ double block_sum256(double acc, double* red4) {
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    return (red4[0] + red4[1]) + (red4[2] + red4[3]);
+}
+
+// Deterministic final sum of n per-workgroup partials (one workgroup of 1024
+// threads, fixed order).  A separate launch: publishing partials across the
+// 8 XCDs from inside one kernel would need a device-scope release per
+// workgroup, which on gfx950 writes back the whole L2 each time.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict__ p, int n, double* out_d, T* out_t) {
     __shared__ double red[16];
@@ -378,6 +507,31 @@ __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict_
         for (int k = 0; k < 16; ++k) s += red[k];
         if (out_d) out_d[0] = s;
         if (out_t) out_t[0] = T(s);
+    }
+}
+
+template <typename T, int MODE, bool SYM>
+__global__ __launch_bounds__(256) void k_sample(SampleArgs<T> a) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    bool oob = false;
+    if (i < a.nd) {
+        const cpx<T> V = SYM ? sample_point_sym(a, a.u[i], a.v[i], oob) : sample_point(a, a.u[i], a.v[i], oob);
+        if constexpr (MODE == MODE_VIS) {
+            a.vis[a.perm ? a.perm[i] : i] = V;
+        } else {
+            const T sw = sqrt(a.w[i]);
+            const T cr = (V.re - a.re[i]) * sw;
+            const T ci = (V.im - a.im[i]) * sw;
+            if constexpr (std::is_same<T, double>::value) acc = cr * cr + ci * ci;
+            else acc = double(cr) * cr + double(ci) * ci;
+        }
+    }
+    if (oob && a.flag) atomicOr(a.flag, 1u);
+    if constexpr (MODE == MODE_CHI2) {
+        const double b = block_sum256(acc, red);
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = b;
     }
 }
 
@@ -422,10 +576,11 @@ __global__ __launch_bounds__(256) void k_apply_phase(int nd, T dRA2, T dDec2, co
     vis[i] = cpx<T>{V.re * c - V.im * s, V.re * s + V.im * c};
 }
 
-// chi2 terms of given visibilities (reduce_chi2, galario.cpp:1506-1538).
+// chi2 terms of given visibilities (reduce_chi2, galario.cpp:1506-1538);
+// `a` carries only the reduction field (partials).
 template <typename T>
 __global__ __launch_bounds__(256) void k_chi2_terms(int nd, const T* re, const T* im, const cpx<T>* vis, const T* w,
-                                                    double* partials) {
+                                                    SampleArgs<T> a) {
     __shared__ double red[4];
     const int i = blockIdx.x * 256 + threadIdx.x;
     double acc = 0.0;
@@ -434,12 +589,11 @@ __global__ __launch_bounds__(256) void k_chi2_terms(int nd, const T* re, const T
         const cpx<T> V = vis[i];
         const T cr = (V.re - re[i]) * sw;
         const T ci = (V.im - im[i]) * sw;
-        acc = std::is_same<T, double>::value ? double(cr * cr + ci * ci) : double(cr) * cr + double(ci) * ci;
+        if constexpr (std::is_same<T, double>::value) acc = cr * cr + ci * ci;
+        else acc = double(cr) * cr + double(ci) * ci;
     }
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) partials[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    const double b = block_sum256(acc, red);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = b;
 }
 
 // Quadrant swap of the N x N real image held in rows of ld reals

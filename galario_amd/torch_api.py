@@ -5,11 +5,19 @@ float32 for the single path); work is enqueued on torch's current stream
 through the C ABI's ``_dev`` functions and nothing is copied to the host.
 Out-of-grid (u, v) points make the results NaN; ``take_oob_flag()``
 synchronises and reports whether that happened since the last call.
+
+``Dataset`` uploads an observation once, in a locality order: the points are
+sorted by the Morton code of their Hermitian-folded position (|u|, sign(u) v),
+so that the threads of a wavefront gather neighbouring spectrum cells (a
+rotation by PA keeps neighbours together except across the new fold line).
+chi2 does not depend on the order; visibilities are returned in the caller's
+order (the kernel writes point i to perm[i]).
 """
 from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from ._lib import check as _check, lib
@@ -58,7 +66,7 @@ def chi2_profile(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_im, vi
 
 
 def sample_profile(intensity, Rmin, dR, nxy, dxy, u, v, dRA=0., dDec=0., PA=0., inc=0., check_central_pixel=True,
-                   out=None):
+                   out=None, perm=None):
     """sampleProfile on device tensors; returns a complex tensor (device)."""
     dt = intensity.dtype
     dev = _check_tensors(dt, intensity, u, v)
@@ -70,7 +78,8 @@ def sample_profile(intensity, Rmin, dR, nxy, dxy, u, v, dRA=0., dDec=0., PA=0., 
     f = getattr(lib, f'galario_sample_profile_dev_{_SUF[dt]}')
     with torch.cuda.device(dev):
         _check(f(_stream_ptr(dev), intensity.numel(), _p(intensity), Rmin, dR, dxy, int(nxy), inc, dRA, dDec,
-                 1.0 / (dxy * nxy), PA, nd, _p(u), _p(v), int(bool(check_central_pixel)), _p(out)))
+                 1.0 / (dxy * nxy), PA, nd, _p(u), _p(v), int(bool(check_central_pixel)),
+                 _p(perm) if perm is not None else None, _p(out)))
     return out
 
 
@@ -81,7 +90,7 @@ def take_oob_flag() -> bool:
     return bool(out.value)
 
 
-STAGES = ('row_pass', 'column_pass', 'sample', 'final_sum')
+STAGES = ('row_pass', 'column_pass', 'sample')
 
 
 def stage_timer(enable: bool) -> None:
@@ -95,3 +104,65 @@ def stage_times():
     n = ctypes.c_int(0)
     _check(lib.galario_stage_times(ms, len(STAGES), ctypes.byref(n)))
     return dict(zip(STAGES, list(ms))), n.value
+
+
+def _spread16(x: np.ndarray) -> np.ndarray:
+    x = x.astype(np.uint64) & 0xFFFF
+    x = (x | (x << 8)) & 0x00FF00FF
+    x = (x | (x << 4)) & 0x0F0F0F0F
+    x = (x | (x << 2)) & 0x33333333
+    x = (x | (x << 1)) & 0x55555555
+    return x
+
+
+def locality_order(u, v) -> np.ndarray:
+    """Permutation sorting (u, v) by the 32-bit Morton code of (|u|, sign(u) v)."""
+    u = np.asarray(u, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    s = np.copysign(1.0, u)
+    fu, fv = np.abs(u), s * v
+    r = max(float(np.max(np.abs(fu), initial=0.0)), float(np.max(np.abs(fv), initial=0.0)), 1e-300)
+    qu = np.clip(fu / r * 65535.0, 0, 65535).astype(np.uint64)
+    qv = np.clip((fv / r + 1.0) * 0.5 * 65535.0, 0, 65535).astype(np.uint64)
+    key = _spread16(qu) | (_spread16(qv) << np.uint64(1))
+    return np.argsort(key, kind='stable')
+
+
+class Dataset:
+    """An observation (u, v [, Re, Im, w]) resident on the GPU in locality order.
+
+    Arrays may be numpy arrays or tensors; they are copied to ``device`` in
+    ``dtype`` (torch.float64 for the double path, torch.float32 for single)."""
+
+    def __init__(self, u, v, vis_obs_re=None, vis_obs_im=None, vis_obs_w=None, device='cuda',
+                 dtype=torch.float64, sort=True):
+        to_np = lambda a: a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)  # noqa: E731
+        u, v = to_np(u).astype(np.float64), to_np(v).astype(np.float64)
+        if u.shape != v.shape or u.ndim != 1:
+            raise ValueError('u and v must be 1-D arrays of the same length')
+        self.n = len(u)
+        order = locality_order(u, v) if sort else np.arange(self.n)
+        dev = torch.device(device)
+        self.device, self.dtype = dev, dtype
+        put = lambda a: torch.from_numpy(np.ascontiguousarray(a[order])).to(dev, dtype)  # noqa: E731
+        self.u, self.v = put(u), put(v)
+        self.has_obs = vis_obs_re is not None
+        if self.has_obs:
+            re, im, w = (to_np(a).astype(np.float64) for a in (vis_obs_re, vis_obs_im, vis_obs_w))
+            if not (re.shape == im.shape == w.shape == u.shape):
+                raise ValueError('vis_obs_re, vis_obs_im, vis_obs_w must have the length of u')
+            self.re, self.im, self.w = put(re), put(im), put(w)
+        self.order = torch.from_numpy(order.astype(np.int32)).to(dev)   # sorted position i <- caller's point order[i]
+
+    def chi2_profile(self, intensity, Rmin, dR, nxy, dxy, dRA=0., dDec=0., PA=0., inc=0.,
+                     check_central_pixel=True, out=None):
+        if not self.has_obs:
+            raise ValueError('this Dataset has no observed visibilities')
+        return chi2_profile(intensity, Rmin, dR, nxy, dxy, self.u, self.v, self.re, self.im, self.w, dRA=dRA,
+                            dDec=dDec, PA=PA, inc=inc, check_central_pixel=check_central_pixel, out=out)
+
+    def sample_profile(self, intensity, Rmin, dR, nxy, dxy, dRA=0., dDec=0., PA=0., inc=0.,
+                       check_central_pixel=True, out=None):
+        """Visibilities in the caller's original point order."""
+        return sample_profile(intensity, Rmin, dR, nxy, dxy, self.u, self.v, dRA=dRA, dDec=dDec, PA=PA, inc=inc,
+                              check_central_pixel=check_central_pixel, out=out, perm=self.order)

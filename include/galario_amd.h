@@ -174,20 +174,23 @@ int galario_chi2_profile_dev_f32(void *stream, int nr, const float *intensity, f
                                  const float *vis_obs_im, const float *weights, int check_central,
                                  double *chi2_d);
 
-/* galario::sample_profile (src/galario.cpp:1404) on device arrays; vis_d: nd complex */
+/* galario::sample_profile (src/galario.cpp:1404) on device arrays; vis_d: nd
+ * complex.  perm (device, may be null): point i is written to vis_d[perm[i]],
+ * so a dataset stored in a locality order returns visibilities in the
+ * caller's order. */
 int galario_sample_profile_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
                                    double dxy, int nxy, double inc, double dRA, double dDec, double duv,
                                    double PA, int nd, const double *u, const double *v, int check_central,
-                                   void *vis_d);
+                                   const int *perm, void *vis_d);
 int galario_sample_profile_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
                                    float dxy, int nxy, float inc, float dRA, float dDec, float duv,
                                    float PA, int nd, const float *u, const float *v, int check_central,
-                                   void *vis_d);
+                                   const int *perm, void *vis_d);
 
 /* Per-stage timing of the calling thread's _dev calls (no counterpart in
  * galario, whose GALARIO_TIMING build prints cudaEvent timings per stage,
  * src/galario.cpp:275-315): enable=1 starts recording a HIP event before each
- * stage (0 row pass, 1 column pass, 2 sample, 3 final sum) and after the last,
+ * stage (0 row pass, 1 column pass, 2 sample + chi2 reduction) and after the last,
  * on the call's stream; enable=0 stops.  Either clears what was recorded. */
 int galario_stage_timer(int enable);
 /* Waits for the recorded events and writes the summed milliseconds of each
