@@ -76,6 +76,8 @@ def _load():
             'reduce_chi2': [_c_int, _vp, _vp, _vp, _vp, _vp],
             'chi2_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp, _vp,
                                  _vp, _c_int, _vp],
+            'chi2_profile_grad_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp,
+                                      _vp, _vp, _c_int, _vp, _vp, _vp],
             'sample_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _c_int,
                                    _vp, _vp],
         }

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "galario_amd.h"
+#include "gl_adjoint.hpp"
 #include "gl_kernels.hpp"
 
 using namespace gl;
@@ -113,13 +114,16 @@ struct Workspace {
     bool has_last = false;        // (the null stream is a valid `last`)
     hipEvent_t ev = nullptr;
     DevBuf Tm, S, part, scal, flag_host, flag_dev;
+    DevBuf Hh, Th;                // gradient: adjoint half spectrum (kept zero between calls), adjoint rows
+    bool Hh_zero = false;         // Hh holds zeros (cleared by the adjoint column pass)
     DevBuf in[8];
     DevBuf out;
     double* h_scal = nullptr;     // pinned: [0] result, [1] flag
     unsigned dev_flag_dirty = 0;
 
     void release() {
-        Tm.release(); S.release(); part.release(); scal.release();
+        Tm.release(); S.release(); part.release(); scal.release(); Hh.release(); Th.release();
+        Hh_zero = false;
         flag_host.release(); flag_dev.release(); out.release();
         for (auto& b : in) b.release();
         if (h_scal) (void)hipHostFree(h_scal);
@@ -135,7 +139,7 @@ thread_local std::map<int, Workspace*> t_ws;
 // When enabled, the _dev entry points record a HIP event before each stage and
 // after the last one, on the stream the kernels run on; galario_stage_times
 // synchronises and returns the summed durations per stage.
-enum { ST_ROW = 0, ST_COL = 1, ST_SAMPLE = 2, NSTAGE = 3 };
+enum { ST_ROW = 0, ST_COL = 1, ST_SAMPLE = 2, ST_ADJCOL = 3, ST_ADJROW = 4, NSTAGE = 5 };
 struct StageTimer {
     bool on = false;
     std::vector<hipEvent_t> pool;            // free events
@@ -284,6 +288,20 @@ void launch_colpass_sym(int L, int blocks, hipStream_t s, const SymColArgs<T>& a
 #undef GL_CALL
 }
 
+template <typename T>
+void launch_adjcol_sym(int L, int blocks, hipStream_t s, const AdjColArgs<T>& a) {
+#define GL_CALL(LL) hipLaunchKernelGGL((k_adjcol_sym<T, LL>), dim3(blocks), dim3(FftShape<LL>::NT), 0, s, a)
+    GL_L_SWITCH(T, GL_CALL)
+#undef GL_CALL
+}
+
+template <typename T>
+void launch_adjrow_sym(int L, int blocks, hipStream_t s, const AdjRowArgs<T>& a) {
+#define GL_CALL(LL) hipLaunchKernelGGL((k_adjrow_sym<T, LL>), dim3(blocks), dim3(FftShape<LL>::NT), 0, s, a)
+    GL_L_SWITCH(T, GL_CALL)
+#undef GL_CALL
+}
+
 template <typename T, int SRC>
 void launch_rowpass(int L, int pairs, hipStream_t s, const RowArgs<T>& a) {
     switch (L) {
@@ -364,30 +382,45 @@ cpx<T>* build_spectrum(Workspace& w, hipStream_t s, RowArgs<T> ra, int N, int Rs
 // Real, even half spectrum H of a swept profile (see k_rowpass_sym); returns
 // false when the workgroup FFT does not cover N (the caller then uses the
 // general complex path).
+struct SymGeom {
+    int L, C, Rs, nrows, pairs_r, pairs_c, ldT, ldH;
+};
+
+SymGeom sym_geom(int L, int N, int rmax) {
+    SymGeom g;
+    g.L = L;
+    g.C = N / 2 + 1;
+    g.Rs = rmax + 1 < N / 2 ? rmax + 1 : N / 2;      // symmetric support half-width
+    g.nrows = g.Rs + 1;
+    g.pairs_r = (g.nrows + 1) / 2;
+    g.pairs_c = (g.C + 1) / 2;
+    g.ldT = 2 * g.pairs_r;                            // column stride of Ts (rows m, even)
+    g.ldH = (N / 2 + 1 + 7) & ~7;
+    return g;
+}
+
+// Real, even half spectrum H of a swept profile (see k_rowpass_sym); returns
+// false when the workgroup FFT does not cover N (the caller then uses the
+// general complex path).  zero/nzero: doubles the row pass clears (gradient
+// accumulators of the same call).
 template <typename T>
 bool build_spectrum_sym(Workspace& w, hipStream_t s, const SweepPrm<T>& sw, int N, int rmax, const T** H_out,
-                        int* ldH_out) {
+                        int* ldH_out, double* zero = nullptr, int nzero = 0, double* zero1 = nullptr) {
     const int L = fast_log2<T>(N);
     if (L < 0) return false;
-    const int C = N / 2 + 1;
-    const int Rs = rmax + 1 < N / 2 ? rmax + 1 : N / 2;      // symmetric support half-width
-    const int nrows = Rs + 1;
-    const int pairs_r = (nrows + 1) / 2;
-    const int ldT = (2 * pairs_r + 1) & ~1;                  // column stride of Ts (rows m, padded even)
-    const int ldH = (N / 2 + 1 + 7) & ~7;
-    T* Ts = w.Tm.get<T>(size_t(C + 1) * ldT);                // C+1 columns: the last pair reads one past C
-    T* H = w.S.get<T>(size_t(C + 1) * ldH);
+    const SymGeom g = sym_geom(L, N, rmax);
+    T* Ts = w.Tm.get<T>(size_t(g.C + 1) * g.ldT);            // C+1 columns: the last pair reads one past C
+    T* H = w.S.get<T>(size_t(g.C + 1) * g.ldH);
     const cpx<T>* tw = twiddles<T>(N);
-    SymRowArgs<T> ra{sw, N, Rs, nrows, ldT, tw, Ts};
-    SymColArgs<T> ca{Ts, N, Rs, ldT, tw, H, ldH};
-    const int pairs_c = (C + 1) / 2;
+    SymRowArgs<T> ra{sw, N, g.Rs, g.nrows, g.ldT, tw, Ts, zero, nzero, zero1};
+    SymColArgs<T> ca{Ts, N, g.Rs, g.ldT, tw, H, g.ldH};
     t_timer.mark(ST_ROW, s);
-    launch_rowpass_sym<T>(L, pairs_r, s, ra);
+    launch_rowpass_sym<T>(L, g.pairs_r, s, ra);
     t_timer.mark(ST_COL, s);
-    launch_colpass_sym<T>(L, 8 * ((pairs_c + 7) / 8), s, ca);
+    launch_colpass_sym<T>(L, 8 * ((g.pairs_c + 7) / 8), s, ca);
     HC(hipGetLastError());
     *H_out = H;
-    *ldH_out = ldH;
+    *ldH_out = g.ldH;
     return true;
 }
 
@@ -889,6 +922,73 @@ int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, in
     });
 }
 
+// chi2 and its gradient with respect to the profile (grad_I, nr doubles) and
+// to inc, PA, dRA, dDec (grad_geom[0..3]), all on the device.
+template <typename T>
+int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec,
+                          T duv, T PA, int nd, const T* u, const T* v, const T* re, const T* im, const T* wt,
+                          int check_central, double* chi2_d, double* grad_I, double* grad_geom) {
+    return guard([&] {
+        check_nd(nd);
+        ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
+        SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        const int L = fast_log2<T>(nxy);
+        if (L < 0)
+            invalid("chi2 gradient", "nxy = " + std::to_string(nxy) +
+                                         " is not supported by the gradient (power of two from 128 to " +
+                                         std::to_string(1 << max_fast_log2<T>()) + ")");
+        if (!grad_I || !grad_geom || !chi2_d) invalid("chi2 gradient", "null output pointer");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        Workspace& w = workspace(s, false);
+        ps.sw.I = I;
+        const SymGeom g = sym_geom(L, nxy, ps.rmax);
+        // Hh is kept zero between calls (the adjoint column pass clears what it read); it is
+        // cleared here only when new or after a call that did not reach that pass
+        const size_t nHh = size_t(g.C + 1) * g.ldH;
+        if (w.Hh.cap < nHh * sizeof(double)) w.Hh_zero = false;
+        double* Hh = w.Hh.get<double>(nHh);
+        if (!w.Hh_zero) {
+            hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(nHh, 256)), dim3(256), 0, s, Hh, nHh);
+            HC(hipGetLastError());
+            w.Hh_zero = true;
+        }
+        const T* H = nullptr;
+        int ldH = 0;
+        // the forward row pass clears the accumulators grad_I[0..nr) and grad_geom[0] (inc)
+        build_spectrum_sym<T>(w, s, ps.sw, nxy, ps.rmax, &H, &ldH, grad_I, nr, grad_geom);
+        SampleArgs<T> sa = sample_args<T>(nullptr, 0, 0, nxy, g.C, T(1), duv, ss, nd, u, v);
+        sa.H = H;
+        sa.ldH = ldH;
+        sa.re = re;
+        sa.im = im;
+        sa.w = wt;
+        sa.flag = w.flag_dev.get<unsigned>(1);
+        const unsigned nb = blocks_for(size_t(nd), 256);
+        double* part = w.part.get<double>(size_t(4) * (nb ? nb : 1));
+        GradArgs<T> ga{Hh, part};
+        t_timer.mark(ST_SAMPLE, s);
+        w.Hh_zero = false;
+        if (nb) hipLaunchKernelGGL((k_sample_grad_sym<T>), dim3(nb), dim3(256), 0, s, sa, ga);
+        Dst4 dst{{chi2_d, grad_geom + 1, grad_geom + 2, grad_geom + 3}};
+        hipLaunchKernelGGL(k_sum_partials4, dim3(1), dim3(1024), 0, s, part, int(nb), dst);
+        HC(hipGetLastError());
+        // adjoint passes
+        const int ldTh = (g.C + 1 + 7) & ~7;
+        T* Th = w.Th.get<T>(size_t(g.nrows) * ldTh);
+        const cpx<T>* tw = twiddles<T>(nxy);
+        AdjColArgs<T> aca{Hh, g.ldH, nxy, g.Rs, tw, Th, ldTh};
+        t_timer.mark(ST_ADJCOL, s);
+        launch_adjcol_sym<T>(L, 8 * ((g.pairs_c + 7) / 8), s, aca);
+        HC(hipGetLastError());
+        w.Hh_zero = true;
+        AdjRowArgs<T> ara{ps.sw, Th, ldTh, nxy, g.Rs, g.nrows, tw, std::tan(double(inc)), grad_I, grad_geom};
+        t_timer.mark(ST_ADJROW, s);
+        launch_adjrow_sym<T>(L, g.pairs_r, s, ara);
+        HC(hipGetLastError());
+        t_timer.end_call(s);
+    });
+}
+
 int g_threads = 16;
 
 }  // namespace
@@ -1003,6 +1103,17 @@ void galario_free(void* ptr) { std::free(ptr); }
     }
 
 GL_BOTH(GL_MAIN)
+
+#define GL_GRAD(SUF, T)                                                                                           \
+    int galario_chi2_profile_grad_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy,       \
+                                            T inc, T dRA, T dDec, T duv, T PA, int nd, const T* u, const T* v,    \
+                                            const T* re, const T* im, const T* w, int cc, double* chi2_d,         \
+                                            double* grad_I, double* grad_geom) {                                  \
+        return chi2_profile_grad_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v, re, \
+                                        im, w, cc, chi2_d, grad_I, grad_geom);                                    \
+    }
+
+GL_BOTH(GL_GRAD)
 
 int galario_stage_timer(int enable) {
     return guard([&] {

@@ -1,0 +1,294 @@
+// Adjoint (reverse-mode) chain of chi2 for the profile path on the real, even
+// spectrum.  galario has no gradient; this differentiates its arithmetic:
+//
+//   V  = sgn(Z) A e^{i theta},  A = sum_k b_k |y_k|,  Z = sum_k b_k y_k
+//        (y_k the four real corners, b_k the bilinear weights), theta =
+//        2 pi (u dRA + v dDec) (the rotated form urot dRArot + vrot dDecrot
+//        is the same number in exact arithmetic, so d theta/dPA = 0)
+//   chi2 = sum w |V - O|^2,  G = dchi2/dV = 2 w (V - O)
+//   dchi2/dA     = sgn(Z) Re(conj(G) e^{i theta})
+//   dchi2/dtheta = Im(G) Re(V) - Re(G) Im(V)
+//   dchi2/dy_k   = dchi2/dA b_k sgn(y_k)          (the phase of a real Z is
+//                                                  locally constant)
+//   PA moves the sampling point: dq/durot = s/duv, dt/dvrot = v_origin s/duv,
+//   durot/dPA = -vrot, dvrot/dPA = urot.
+//
+// The corner gradients are scattered with FP64 atomics into Hh, the adjoint
+// of the half spectrum H (both mirror rows of a frequency land on the same
+// element, which is the adjoint of reading H at |r - N/2|).  The two passes
+// are then transposed: each is sum_k c[k] cos(2 pi k m / N) over an even
+// sequence, so its adjoint is the same even-real transform of the weighted
+// even extension (weight 1/2 on interior input frequencies) times the
+// multiplicity of each output index (1 at 0 and N/2, 2 elsewhere).  The row
+// adjoint ends in the sweep adjoint: every pixel feeds I[iR] with weight
+// sr (1 - alpha) and I[iR+1] with sr alpha, and inc through dr/dinc =
+// (x/cos inc)^2 tan(inc) / r; the central pixel is linear in I[0..iIN+1].
+#pragma once
+
+#include "gl_kernels.hpp"
+
+namespace gl {
+
+__device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------------------------------
+// sample + its adjoint: chi2 and (PA, dRA, dDec) partials per workgroup,
+// corner gradients scattered into Hh.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct GradArgs {
+    double* Hh;          // adjoint of H, same layout (column stride ldH), zero on entry
+    double* partials;    // 4 x nblocks: chi2, dPA, ddRA, ddDec
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sample_grad_sym(SampleArgs<T> a, GradArgs<T> g) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double c2 = 0.0, gpa = 0.0, gra = 0.0, gde = 0.0;
+    bool oob = false;
+    if (i < a.nd) {
+        const T u0 = a.u[i], v0 = a.v[i];
+        const GridPos<T> p = grid_pos(a, u0, v0);
+        oob = p.oob;
+        const T q = p.q, t = p.t;
+        const int half = a.nrow / 2;
+        const int m0 = abs(p.fl_v - half), m1 = abs(p.fv1 - half);
+        const size_t i00 = size_t(p.fl_u) * a.ldH + m0, i01 = size_t(p.fl_u) * a.ldH + m1;
+        const size_t i10 = size_t(p.fu1) * a.ldH + m0, i11 = size_t(p.fu1) * a.ldH + m1;
+        const T y0 = a.H[i00], y1 = a.H[i01], y3 = a.H[i10], y2 = a.H[i11];
+        const T z = (t * (q * ((y0 + y2) - (y1 + y3))) + (t * (y1 - y0) + q * (y3 - y0))) + y0;
+        const T a0 = fabs(y0), a1 = fabs(y1), a2 = fabs(y2), a3 = fabs(y3);
+        T amp = a0;
+        amp += (a3 - a0) * q;
+        amp += (a1 - a0) * t;
+        amp += (((a0 - a1) + a2) - a3) * t * q;
+        const T sg = copysign(T(1), z);
+        T cth = T(1), sth = T(0);
+        if (a.phase) {
+            const T ang = p.urot * a.dRA2 + p.vrot * a.dDec2;
+            fsincos(ang, &sth, &cth);
+        }
+        const T V0 = copysign(amp, z);
+        const T Vre = V0 * cth, Vim = V0 * sth;
+        const T w = a.w[i];
+        const T sw = sqrt(w);
+        const T cr = (Vre - a.re[i]) * sw, ci = (Vim - a.im[i]) * sw;
+        if constexpr (std::is_same<T, double>::value) c2 = cr * cr + ci * ci;
+        else c2 = double(cr) * cr + double(ci) * ci;
+        // reverse mode in double
+        const double Gre = 2.0 * double(w) * (double(Vre) - double(a.re[i]));
+        const double Gim = 2.0 * double(w) * (double(Vim) - double(a.im[i]));
+        const double gA = double(sg) * (Gre * double(cth) + Gim * double(sth));
+        const double gth = Gim * double(Vre) - Gre * double(Vim);
+        const double dq = double(1 - t) * double(a3 - a0) + double(t) * double(a2 - a1);
+        const double dt = double(1 - q) * double(a1 - a0) + double(q) * double(a2 - a3);
+        const double s_duv = double(p.sign_u) / double(a.duv);
+        gpa = gA * (dq * s_duv * (-double(p.vrot)) + dt * double(a.v_origin) * s_duv * double(p.urot));
+        const double two_pi = 6.283185307179586476925286766559;
+        gra = gth * two_pi * double(u0);
+        gde = gth * two_pi * double(v0);
+        if (!oob) {
+            const double b0 = double(1 - t) * double(1 - q), b1 = double(t) * double(1 - q);
+            const double b2 = double(t) * double(q), b3 = double(1 - t) * double(q);
+            const auto sgn = [](T y) { return double((y > T(0)) - (y < T(0))); };
+            const double g0 = gA * b0 * sgn(y0), g1 = gA * b1 * sgn(y1);
+            const double g2 = gA * b2 * sgn(y2), g3 = gA * b3 * sgn(y3);
+            if (g0 != 0.0) atomic_add_f64(g.Hh + i00, g0);
+            if (g1 != 0.0) atomic_add_f64(g.Hh + i01, g1);
+            if (g3 != 0.0) atomic_add_f64(g.Hh + i10, g3);
+            if (g2 != 0.0) atomic_add_f64(g.Hh + i11, g2);
+        } else {
+            c2 = gpa = gra = gde = double(NAN);
+        }
+    }
+    if (oob && a.flag) atomicOr(a.flag, 1u);
+    const unsigned nb = gridDim.x;
+    double* P = g.partials;
+    double v;
+    v = block_sum256(c2, red);
+    if (threadIdx.x == 0) P[blockIdx.x] = v;
+    __syncthreads();
+    v = block_sum256(gpa, red);
+    if (threadIdx.x == 0) P[nb + blockIdx.x] = v;
+    __syncthreads();
+    v = block_sum256(gra, red);
+    if (threadIdx.x == 0) P[2 * nb + blockIdx.x] = v;
+    __syncthreads();
+    v = block_sum256(gde, red);
+    if (threadIdx.x == 0) P[3 * nb + blockIdx.x] = v;
+}
+
+// Deterministic sums of 4 streams of n partials: *dst[j] = sum_i p[j*n + i].
+struct Dst4 {
+    double* d[4];
+};
+
+__global__ __launch_bounds__(1024) void k_sum_partials4(const double* __restrict__ p, int n, Dst4 dst) {
+    __shared__ double red[16];
+    for (int j = 0; j < 4; ++j) {
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < n; i += 1024) acc += p[size_t(j) * n + i];
+        acc = wave_sum(acc);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int w = 0; w < 16; ++w) s += red[w];
+            if (dst.d[j]) *dst.d[j] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of the column pass: Hh (two columns per workgroup) -> Th[m][l],
+// m = 0..Rs'; Hh is set back to zero after it is read.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct AdjColArgs {
+    double* Hh;
+    int ldH;
+    int N, Rs;           // Rs': outputs m = 0..Rs'
+    const cpx<T>* tw;
+    T* Th;               // row-major [Rs'+1][ldTh]
+    int ldTh;
+};
+
+__device__ __forceinline__ double mult(int m, int N) { return (m == 0 || m == N / 2) ? 1.0 : 2.0; }
+__device__ __forceinline__ double ext_weight(int k, int N) { return (k == 0 || k == N / 2) ? 1.0 : 0.5; }
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym(AdjColArgs<T> a) {
+    using S = FftShape<LOG2N>;
+    using F = BlockFft<T, LOG2N, false>;
+    constexpr int N = S::N, NT = S::NT, E = S::E;
+    constexpr int C = N / 2 + 1;
+    constexpr int NPAIR = (C + 1) / 2;
+    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    const int pair = xcd_column(blockIdx.x, gridDim.x);
+    if (pair >= NPAIR) return;
+    const int la = 2 * pair, lb = la + 1;
+    const bool has_b = lb < C;
+    const int tid = threadIdx.x;
+    double* ha = a.Hh + size_t(la) * a.ldH;
+    double* hb = ha + a.ldH;
+    cpx<T> v[E];
+    F::run(v, lds, tid, N / 2, a.tw, [&](int k) -> cpx<T> {
+        const int p = k <= N / 2 ? k : N - k;
+        const double wk = ext_weight(k, N);
+        return cpx<T>{T(wk * ha[p]), has_b ? T(wk * hb[p]) : T(0)};
+    });
+    // all reads of Hh are done (the FFT synchronised after its first pass): clear it for the next call
+    for (int p = tid; p <= N / 2; p += NT) {
+        ha[p] = 0.0;
+        if (has_b) hb[p] = 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int m = tid + r * NT;
+        if (r * NT <= a.Rs && m <= a.Rs) {
+            const T cm = T(mult(m, N));
+            T* o = a.Th + size_t(m) * a.ldTh + la;
+            o[0] = cm * v[r].re;
+            if (has_b) o[1] = cm * v[r].im;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of the row pass (two rows per workgroup) fused with the sweep
+// adjoint: Th rows -> pixel gradients -> dchi2/dI (atomics) and dchi2/dinc.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct AdjRowArgs {
+    SweepPrm<T> sw;
+    const T* Th;
+    int ldTh;
+    int N, Rs, nrows;    // pixels j = 0..Rs', rows m = 0..nrows-1
+    const cpx<T>* tw;
+    double tan_inc;
+    double* gI;          // nr, zeroed by the forward row pass
+    double* ginc;        // 1, zeroed by the forward row pass
+};
+
+// dchi2/d(pixel value) = gz: accumulate into dI and return the inc term.
+template <typename T>
+__device__ __forceinline__ double sweep_pixel_adjoint(int xi, int yi, double gz, const SweepPrm<T>& p, double tan_inc,
+                                                      double* gI) {
+    if (gz == 0.0) return 0.0;
+    const T x = T(xi) * p.dxy;
+    const T y = T(yi) * p.dxy;
+    T r;
+    double xs2;
+    if constexpr (std::is_same<T, double>::value) {
+        const double xs = x / p.cos_inc;
+        xs2 = xs * xs;
+        r = sqrt(xs * xs + y * y);
+    } else {
+        const double xs = double(x / p.cos_inc);
+        xs2 = xs * xs;
+        r = float(sqrt(xs * xs + double(y) * double(y)));
+    }
+    T fi = floor((r - p.Rmin) / p.dR);
+    fi = fi > T(0) ? fi : T(0);
+    if (!(fi <= T(p.nr - 2))) return 0.0;
+    const int iR = int(fi);
+    const double alpha = (double(r) - double(iR) * double(p.dR) - double(p.Rmin)) / double(p.dR);
+    const double sr = double(p.sr_to_px);
+    atomic_add_f64(gI + iR, gz * sr * (1.0 - alpha));
+    atomic_add_f64(gI + iR + 1, gz * sr * alpha);
+    if (r == T(0)) return 0.0;
+    const double dvdr = sr * (double(p.I[iR + 1]) - double(p.I[iR])) / double(p.dR);
+    return gz * dvdr * xs2 * tan_inc / double(r);
+}
+
+template <typename T>
+__device__ __noinline__ void central_adjoint(double gz, const SweepPrm<T>& p, double* gI) {
+    const int iIN = p.iIN;
+    const double Rmin = p.Rmin, dR = p.dR, h = double(p.dxy) / 2.0;
+    const double area = h * h - Rmin * Rmin;
+    const double sc = gz * double(p.sr_to_px) / area;
+    const double RiIN = Rmin + iIN * dR;
+    const double delta = h - RiIN, beta = delta / dR;
+    atomic_add_f64(gI + 0, sc * dR * Rmin);
+    for (int i = 1; i < iIN; ++i) atomic_add_f64(gI + i, sc * 2.0 * dR * (Rmin + i * dR));
+    atomic_add_f64(gI + iIN, sc * (dR * RiIN + delta * (RiIN + h * (1.0 - beta))));
+    atomic_add_f64(gI + iIN + 1, sc * delta * h * beta);
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjrow_sym(AdjRowArgs<T> a) {
+    using S = FftShape<LOG2N>;
+    using F = BlockFft<T, LOG2N, false>;
+    constexpr int N = S::N, NT = S::NT, E = S::E;
+    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    __shared__ double red[16];
+    const int tid = threadIdx.x;
+    const int ma = 2 * blockIdx.x, mb = ma + 1;
+    const bool hb = mb < a.nrows;
+    const T* ra = a.Th + size_t(ma) * a.ldTh;
+    const T* rb = ra + a.ldTh;
+    cpx<T> v[E];
+    F::run(v, lds, tid, N / 2, a.tw, [&](int k) -> cpx<T> {
+        const int l = k <= N / 2 ? k : N - k;
+        const T wk = T(ext_weight(k, N));
+        return cpx<T>{wk * ra[l], hb ? wk * rb[l] : T(0)};
+    });
+    double ginc = 0.0;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int j = tid + r * NT;
+        if (r * NT <= a.Rs && j <= a.Rs) {
+            const double cj = mult(j, N);
+            const double gza = cj * double(v[r].re), gzb = cj * double(v[r].im);
+            if (ma == 0 && j == 0) central_adjoint(gza, a.sw, a.gI);
+            else ginc += sweep_pixel_adjoint(j, ma, gza, a.sw, a.tan_inc, a.gI);
+            if (hb) ginc += sweep_pixel_adjoint(j, mb, gzb, a.sw, a.tan_inc, a.gI);
+        }
+    }
+    const double b = block_sum_any(ginc, red);
+    if (tid == 0 && b != 0.0) atomic_add_f64(a.ginc, b);
+}
+
+}  // namespace gl

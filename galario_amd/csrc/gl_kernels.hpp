@@ -291,6 +291,9 @@ struct SymRowArgs {
     int ldT;              // column stride of Ts (>= 2 * number of row pairs)
     const cpx<T>* tw;
     T* Ts;
+    double* zero;         // optional: workgroup 0 clears zero[0..nzero) and *zero1 (gradient accumulators)
+    int nzero;
+    double* zero1;
 };
 
 template <typename T, int LOG2N>
@@ -303,6 +306,11 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass_sym(SymRowArgs<
     const int tid = threadIdx.x;
     const int ma = 2 * blockIdx.x, mb = ma + 1;
     const bool hb = mb < a.nrows;
+    if (blockIdx.x == 0) {
+        if (a.zero)
+            for (int k = tid; k < a.nzero; k += NT) a.zero[k] = 0.0;
+        if (a.zero1 && tid == 0) *a.zero1 = 0.0;
+    }
     cpx<T> v[E];
     F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
         const int xi = n < N / 2 ? -n : N - n;
@@ -482,6 +490,18 @@ __device__ __forceinline__ cpx<T> sample_point_sym(const SampleArgs<T>& a, T u0,
     return V;
 }
 
+// Deterministic sum over a workgroup of any size <= 1024 (red: 16 doubles);
+// every thread gets the result.
+__device__ __forceinline__ double block_sum_any(double acc, double* red16) {
+    acc = wave_sum(acc);
+    const int nw = int((blockDim.x + 63) >> 6);
+    if ((threadIdx.x & 63) == 0) red16[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < nw; ++k) s += red16[k];
+    return s;
+}
+
 // Deterministic block sum of `acc` (blockDim 256) into *out on thread 0.
 __device__ __forceinline__ double block_sum256(double acc, double* red4) {
     acc = wave_sum(acc);
@@ -594,6 +614,12 @@ __global__ __launch_bounds__(256) void k_chi2_terms(int nd, const T* re, const T
     }
     const double b = block_sum256(acc, red);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = b;
+}
+
+// Clears n doubles (a kernel rather than a memset, so that it may sit in a captured graph).
+__global__ __launch_bounds__(256) void k_fill_zero(double* p, size_t n) {
+    const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0;
 }
 
 // Quadrant swap of the N x N real image held in rows of ld reals

@@ -90,7 +90,7 @@ def take_oob_flag() -> bool:
     return bool(out.value)
 
 
-STAGES = ('row_pass', 'column_pass', 'sample')
+STAGES = ('row_pass', 'column_pass', 'sample', 'adjoint_column_pass', 'adjoint_row_pass')
 
 
 def stage_timer(enable: bool) -> None:
@@ -166,3 +166,71 @@ class Dataset:
         """Visibilities in the caller's original point order."""
         return sample_profile(intensity, Rmin, dR, nxy, dxy, self.u, self.v, dRA=dRA, dDec=dDec, PA=PA, inc=inc,
                               check_central_pixel=check_central_pixel, out=out, perm=self.order)
+
+
+def chi2_profile_grad(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_im, vis_obs_w, dRA=0., dDec=0., PA=0.,
+                      inc=0., check_central_pixel=True):
+    """chi2 and its gradient on device tensors.
+
+    Returns (chi2 [1], grad_intensity [nr], grad_geom [4] = d/d(inc, PA, dRA, dDec)), float64 device tensors."""
+    dt = intensity.dtype
+    dev = _check_tensors(dt, intensity, u, v, vis_obs_re, vis_obs_im, vis_obs_w)
+    nd = u.numel()
+    if not (v.numel() == nd == vis_obs_re.numel() == vis_obs_im.numel() == vis_obs_w.numel()):
+        raise ValueError('u, v, vis_obs_re, vis_obs_im, vis_obs_w must have the same length')
+    chi2 = torch.empty(1, dtype=torch.float64, device=dev)
+    gI = torch.empty(intensity.numel(), dtype=torch.float64, device=dev)
+    gg = torch.empty(4, dtype=torch.float64, device=dev)
+    f = getattr(lib, f'galario_chi2_profile_grad_dev_{_SUF[dt]}')
+    with torch.cuda.device(dev):
+        _check(f(_stream_ptr(dev), intensity.numel(), _p(intensity), Rmin, dR, dxy, int(nxy), inc, dRA, dDec,
+                 1.0 / (dxy * nxy), PA, nd, _p(u), _p(v), _p(vis_obs_re), _p(vis_obs_im), _p(vis_obs_w),
+                 int(bool(check_central_pixel)), _p(chi2), _p(gI), _p(gg)))
+    return chi2, gI, gg
+
+
+def _dataset_grad(self, intensity, Rmin, dR, nxy, dxy, dRA=0., dDec=0., PA=0., inc=0., check_central_pixel=True):
+    if not self.has_obs:
+        raise ValueError('this Dataset has no observed visibilities')
+    return chi2_profile_grad(intensity, Rmin, dR, nxy, dxy, self.u, self.v, self.re, self.im, self.w, dRA=dRA,
+                             dDec=dDec, PA=PA, inc=inc, check_central_pixel=check_central_pixel)
+
+
+Dataset.chi2_profile_grad = _dataset_grad
+
+
+class Chi2ProfileFunction(torch.autograd.Function):
+    """chi2 of a radial profile against a Dataset, differentiable with respect to
+    the profile and to geom = (inc, PA, dRA, dDec).
+
+    The forward pass computes the gradient together with chi2 when any input
+    needs it (one fused device call), and the backward pass scales it by the
+    incoming gradient.  geom values are read on the host (a device tensor is
+    synchronised)."""
+
+    @staticmethod
+    def forward(ctx, intensity, geom, dataset, Rmin, dR, nxy, dxy, check_central_pixel):
+        inc, PA, dRA, dDec = (float(x) for x in geom.detach().cpu().tolist())
+        I = intensity.detach().contiguous()
+        kw = dict(dRA=dRA, dDec=dDec, PA=PA, inc=inc, check_central_pixel=check_central_pixel)
+        if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
+            chi2, gI, gg = dataset.chi2_profile_grad(I, Rmin, dR, nxy, dxy, **kw)
+            ctx.save_for_backward(gI, gg)
+            ctx.dtypes = (intensity.dtype, geom.dtype, geom.device)
+        else:
+            chi2 = dataset.chi2_profile(I, Rmin, dR, nxy, dxy, **kw)
+        return chi2.reshape(())
+
+    @staticmethod
+    def backward(ctx, gout):
+        gI, gg = ctx.saved_tensors
+        idt, gdt, gdev = ctx.dtypes
+        dI = (gout * gI).to(idt) if ctx.needs_input_grad[0] else None
+        dg = (gout * gg).to(device=gdev, dtype=gdt) if ctx.needs_input_grad[1] else None
+        return dI, dg, None, None, None, None, None, None
+
+
+def chi2_profile_autograd(dataset, intensity, geom, Rmin, dR, nxy, dxy, check_central_pixel=True):
+    """Differentiable chi2Profile: intensity (nr) on the dataset's device, geom = tensor
+    (inc, PA, dRA, dDec) in radians.  Returns a 0-dim float64 tensor."""
+    return Chi2ProfileFunction.apply(intensity, geom, dataset, Rmin, dR, nxy, dxy, check_central_pixel)

@@ -187,10 +187,28 @@ int galario_sample_profile_dev_f32(void *stream, int nr, const float *intensity,
                                    float PA, int nd, const float *u, const float *v, int check_central,
                                    const int *perm, void *vis_d);
 
+/* chi2 of a profile and its gradient (no counterpart in galario): *chi2_d,
+ * grad_I[0..nr) = dchi2/dintensity, grad_geom[0..3] = dchi2/d(inc, PA, dRA,
+ * dDec), all device doubles, enqueued on `stream`.  Requires nxy a power of
+ * two in [128, 8192] (double) or [128, 16384] (float).  dchi2/dintensity is
+ * accumulated with atomics (summation order, hence the last bits, may vary
+ * between calls). */
+int galario_chi2_profile_grad_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
+                                      double dxy, int nxy, double inc, double dRA, double dDec, double duv,
+                                      double PA, int nd, const double *u, const double *v,
+                                      const double *vis_obs_re, const double *vis_obs_im, const double *weights,
+                                      int check_central, double *chi2_d, double *grad_I, double *grad_geom);
+int galario_chi2_profile_grad_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
+                                      float dxy, int nxy, float inc, float dRA, float dDec, float duv,
+                                      float PA, int nd, const float *u, const float *v,
+                                      const float *vis_obs_re, const float *vis_obs_im, const float *weights,
+                                      int check_central, double *chi2_d, double *grad_I, double *grad_geom);
+
 /* Per-stage timing of the calling thread's _dev calls (no counterpart in
  * galario, whose GALARIO_TIMING build prints cudaEvent timings per stage,
  * src/galario.cpp:275-315): enable=1 starts recording a HIP event before each
- * stage (0 row pass, 1 column pass, 2 sample + chi2 reduction) and after the last,
+ * stage (0 row pass, 1 column pass, 2 sample + chi2 reduction, 3 adjoint column
+ * pass, 4 adjoint row pass + profile adjoint) and after the last,
  * on the call's stream; enable=0 stops.  Either clears what was recorded. */
 int galario_stage_timer(int enable);
 /* Waits for the recorded events and writes the summed milliseconds of each

@@ -1,0 +1,197 @@
+"""GPU: the hand-written adjoint chain of chi2Profile.
+
+Checked against
+  * the torch composition of galario's forward math (tests/torch_reference.py,
+    CPU float64 autograd on the complex spectrum, itself pinned to oracle
+    finite differences): relative 1e-7 for the profile gradient (norm-wise),
+    1e-8 for each geometry gradient;
+  * central finite differences of the GPU forward away from kinks, relative
+    1e-6: galario's amplitude/phase interpolation jumps from +A to -A where
+    the interpolated (real) spectrum changes sign, so points whose sign
+    differs between the two evaluations are dropped; for PA (which moves the
+    points) so are points whose grid cell changes, since the bilinear slope
+    jumps at cell edges.  Step 1e-6 relative, except inc: inc moves every
+    pixel radius across the knots of the profile's linear interpolation,
+    where the slope jumps, which gives an O(h) error in a central difference
+    (2.5e-5 at h = 1e-6); at h = 1e-8 it is ~2e-7;
+  * repeat calls (the adjoint grid is cleared between calls): chi2 identical,
+    gradients to 1e-12 (atomic accumulation order);
+  * the float path against the double path: relative 1e-3 for chi2 and the
+    profile gradient, 1e-2 for the geometry gradients (float positions carry
+    ~1e-5 absolute error in grid units).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import galario_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip('torch')
+import torch_reference as TR  # noqa: E402
+
+arcsec = O.ARCSEC
+DEV = 'cuda:0'
+
+
+def problem(seed=3, n=1500, N=128, nr=200, dR=0.004 * arcsec, dxy=0.02 * arcsec, frac=0.4):
+    rng = np.random.default_rng(seed)
+    Rmin = 1e-4 * arcsec
+    R = Rmin + dR * np.arange(nr)
+    I = 1e10 * (np.exp(-((R - 0.3 * arcsec) / (0.12 * arcsec)) ** 2) + 0.3 * np.exp(-R / (0.3 * arcsec)))
+    duv = 1 / (N * dxy)
+    b = rng.uniform(1.5, frac * N, n) * duv
+    th = rng.uniform(0, 2 * np.pi, n)
+    u, v = b * np.cos(th), b * np.sin(th)
+    geom = np.array([0.6, 1.1, 0.03 * arcsec, -0.02 * arcsec])
+    vis = O.sample_profile(I, Rmin, dR, dxy, N, geom[0], geom[2], geom[3], duv, geom[1], u, v, check_central=False)
+    sig = np.median(np.abs(vis))
+    re = vis.real + sig * rng.standard_normal(n)
+    im = vis.imag + sig * rng.standard_normal(n)
+    w = np.full(n, 1 / sig ** 2) * rng.uniform(0.5, 2.0, n)
+    return dict(I=I, Rmin=Rmin, dR=dR, N=N, dxy=dxy, u=u, v=v, re=re, im=im, w=w, geom=geom)
+
+
+def gpu_grad(d, dtype=torch.float64, ds=None):
+    from galario_amd import torch_api as ta
+    if ds is None:
+        ds = ta.Dataset(d['u'], d['v'], d['re'], d['im'], d['w'], device=DEV, dtype=dtype)
+    I = torch.tensor(d['I'], dtype=dtype, device=DEV)
+    g = d['geom']
+    chi2, gI, gg = ds.chi2_profile_grad(I, d['Rmin'], d['dR'], d['N'], d['dxy'], inc=g[0], PA=g[1], dRA=g[2],
+                                        dDec=g[3], check_central_pixel=False)
+    return chi2.item(), gI.cpu().numpy(), gg.cpu().numpy()
+
+
+def gpu_chi2(d, I=None, geom=None, ds=None):
+    from galario_amd import torch_api as ta
+    I = d['I'] if I is None else I
+    g = d['geom'] if geom is None else geom
+    ds = ds or ta.Dataset(d['u'], d['v'], d['re'], d['im'], d['w'], device=DEV)
+    c = ds.chi2_profile(torch.tensor(I, device=DEV), d['Rmin'], d['dR'], d['N'], d['dxy'], inc=g[0], PA=g[1],
+                        dRA=g[2], dDec=g[3], check_central_pixel=False)
+    return c.item()
+
+
+def torch_ref_grad(d):
+    T = lambda a: torch.tensor(a, dtype=torch.float64)  # noqa: E731
+    I = T(d['I']).requires_grad_(True)
+    g = T(d['geom']).requires_grad_(True)
+    c = TR.chi2_profile(I, g, d['Rmin'], d['dR'], d['N'], d['dxy'], T(d['u']), T(d['v']), T(d['re']), T(d['im']),
+                        T(d['w']))
+    c.backward()
+    return float(c), I.grad.numpy(), g.grad.numpy()
+
+
+@pytest.mark.parametrize("N, n", [(128, 1500), (256, 4000), (1024, 20000)])
+def test_gradient_vs_torch_reference(N, n):
+    d = problem(N=N, n=n, dxy=0.02 * arcsec * 128 / N, dR=0.004 * arcsec * 128 / N, nr=200 * N // 128)
+    c, gI, gg = gpu_grad(d)
+    c_ref, gI_ref, gg_ref = torch_ref_grad(d)
+    print(f"\nN={N} chi2 {c:.12e} ref {c_ref:.12e}; |dI - ref|/|ref| = "
+          f"{np.linalg.norm(gI - gI_ref) / np.linalg.norm(gI_ref):.2e}; geom {gg} ref {gg_ref}")
+    assert math.isclose(c, c_ref, rel_tol=1e-11)
+    assert np.linalg.norm(gI - gI_ref) <= 1e-7 * np.linalg.norm(gI_ref)
+    for k in range(4):
+        assert math.isclose(gg[k], gg_ref[k], rel_tol=1e-8), (k, gg[k], gg_ref[k])
+
+
+def _signs(ds, d, I, geom):
+    """sign of the real interpolated spectrum at each point: V e^{-i theta} = sgn(Z) A."""
+    vis = ds.sample_profile(torch.tensor(I, device=DEV), d['Rmin'], d['dR'], d['N'], d['dxy'], inc=geom[0],
+                            PA=geom[1], dRA=geom[2], dDec=geom[3], check_central_pixel=False).cpu().numpy()
+    theta = 2 * np.pi * (d['u'] * geom[2] + d['v'] * geom[3])
+    return np.sign((vis * np.exp(-1j * theta)).real)
+
+
+def _cells(d, PA):
+    """grid cell (floor of the fractional indices) of each point at position angle PA."""
+    duv = 1 / (d['N'] * d['dxy'])
+    urot = d['u'] * np.cos(PA) - d['v'] * np.sin(PA)
+    vrot = d['u'] * np.sin(PA) + d['v'] * np.cos(PA)
+    s = np.copysign(1.0, urot)
+    return np.stack([np.floor(np.abs(urot) / duv), np.floor(d['N'] // 2 + s * vrot / duv)])
+
+
+def test_gradient_vs_finite_differences():
+    """Central differences of the GPU forward, step 1e-6 relative, away from kinks."""
+    from galario_amd import torch_api as ta
+    d = problem(N=1024, n=20000, dxy=0.0025 * arcsec, dR=0.0005 * arcsec, nr=1600, seed=11)
+    # taper the profile to zero over its last 10% of bins: galario's sweep drops pixels beyond the last
+    # bin (iR > nr-2 -> 0), a jump of sr*I[nr-1] that a central difference in inc would pick up
+    nt = len(d['I']) // 10
+    d['I'] = d['I'].copy()
+    d['I'][-nt:] *= 0.5 * (1 + np.cos(np.linspace(0, np.pi, nt)))
+    ds_all = ta.Dataset(d['u'], d['v'], d['re'], d['im'], d['w'], device=DEV, sort=False)
+    # a smooth direction: a per-bin random one is broadband in frequency and moves spectrum
+    # corners at round-off level across zero, where |y| has a kink
+    R = np.arange(len(d['I']))
+    dirI = d['I'] * (np.sin(R / 37.0) + 0.5 * np.cos(R / 91.0))
+    h = 1e-6
+    cases = [('profile', d['I'] + h * dirI, d['I'] - h * dirI, d['geom'], d['geom'], 2 * h)]
+    for k in range(4):
+        hk = (1e-8 if k == 0 else 1e-6) * abs(d['geom'][k])
+        gp, gm = d['geom'].copy(), d['geom'].copy()
+        gp[k] += hk
+        gm[k] -= hk
+        cases.append((f'geom[{k}]', d['I'], d['I'], gp, gm, 2 * hk))
+    keep = np.ones(len(d['u']), bool)
+    for _, Ip, Im, gp, gm, _ in cases:
+        keep &= _signs(ds_all, d, Ip, gp) == _signs(ds_all, d, Im, gm)
+    nsign = int((~keep).sum())
+    pa_p, pa_m = cases[2][3][1], cases[2][4][1]
+    keep &= np.all(_cells(d, pa_p) == _cells(d, pa_m), axis=0)
+    print(f"\n{nsign} of {keep.size} points change sign, {int((~keep).sum()) - nsign} more change cell; dropped")
+    assert keep.mean() > 0.99
+    sub = {k: (d[k][keep] if k in ('u', 'v', 're', 'im', 'w') else d[k]) for k in d}
+    ds = ta.Dataset(sub['u'], sub['v'], sub['re'], sub['im'], sub['w'], device=DEV)
+    c, gI, gg = gpu_grad(sub, ds=ds)
+    analytic = [float(gI @ dirI)] + list(gg)
+    bad = []
+    for (name, Ip, Im, gp, gm, den), an in zip(cases, analytic):
+        fd = (gpu_chi2(sub, I=Ip, geom=gp, ds=ds) - gpu_chi2(sub, I=Im, geom=gm, ds=ds)) / den
+        print(f"{name}: fd {fd:.10e} analytic {an:.10e} rel {abs(fd - an) / abs(an):.2e}")
+        if not math.isclose(fd, an, rel_tol=1e-6):
+            bad.append(name)
+    assert not bad, bad
+
+
+def test_repeat_calls_and_forward_consistency():
+    from galario_amd import torch_api as ta
+    d = problem(N=256, n=4000)
+    ds = ta.Dataset(d['u'], d['v'], d['re'], d['im'], d['w'], device=DEV)
+    c1, gI1, gg1 = gpu_grad(d, ds=ds)
+    c2, gI2, gg2 = gpu_grad(d, ds=ds)
+    assert c1 == c2 == gpu_chi2(d, ds=ds)
+    assert np.linalg.norm(gI1 - gI2) <= 1e-12 * np.linalg.norm(gI1)
+    np.testing.assert_allclose(gg1, gg2, rtol=1e-12)
+
+
+def test_single_precision_gradient():
+    d = problem(N=256, n=4000)
+    c64, gI64, gg64 = gpu_grad(d, torch.float64)
+    c32, gI32, gg32 = gpu_grad(d, torch.float32)
+    assert math.isclose(c32, c64, rel_tol=1e-3)
+    assert np.linalg.norm(gI32 - gI64) <= 1e-3 * np.linalg.norm(gI64)
+    np.testing.assert_allclose(gg32, gg64, rtol=1e-2)
+
+
+def test_autograd_function():
+    from galario_amd import torch_api as ta
+    d = problem(N=256, n=4000)
+    ds = ta.Dataset(d['u'], d['v'], d['re'], d['im'], d['w'], device=DEV)
+    I = torch.tensor(d['I'], device=DEV, requires_grad=True)
+    geom = torch.tensor(d['geom'], dtype=torch.float64, requires_grad=True)
+    loss = 0.5 * ta.chi2_profile_autograd(ds, I, geom, d['Rmin'], d['dR'], d['N'], d['dxy'],
+                                          check_central_pixel=False)
+    loss.backward()
+    c, gI, gg = gpu_grad(d, ds=ds)
+    assert math.isclose(float(loss), 0.5 * c, rel_tol=1e-15)
+    assert np.linalg.norm(I.grad.cpu().numpy() - 0.5 * gI) <= 1e-12 * np.linalg.norm(gI)
+    np.testing.assert_allclose(geom.grad.numpy(), 0.5 * gg, rtol=1e-12)
+    with torch.no_grad():
+        c_ng = ta.chi2_profile_autograd(ds, I, geom, d['Rmin'], d['dR'], d['N'], d['dxy'],
+                                        check_central_pixel=False)
+    assert float(c_ng) == c
