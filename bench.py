@@ -30,21 +30,30 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "chi2Profile evals/s fwd & fwd+grad, nxy=4096 Nvis=1e6 FP64; % HBM roofline"
 
 
-def stage_bytes(nxy, nvis, nr, rmax, s=8):
+def stage_bytes(nxy, nvis, nr, rmax, grad=False, s=8):
     """Algorithmic HBM bytes per launch of each stage of the profile path (each
     needed array touched once).  The swept image is even in x and y, so its
     spectrum is real and even: the row pass writes Rs'+1 real rows
     (Rs' = min(rmax+1, nxy/2)), the column pass reads them and writes the
-    (nxy/2+1) x (nxy/2+1) real half spectrum, the sample kernel streams
-    u, v, Re, Im, w and gathers 4 real corners per point."""
+    (nxy/2+1) x (nxy/2+1) real half spectrum H, the sample kernel streams
+    u, v, Re, Im, w and gathers 4 real corners per point.  With the gradient
+    it also scatters 4 corner gradients (FP64 atomics) into the adjoint Hh;
+    the adjoint column pass reads Hh, clears it and writes Rs'+1 adjoint rows;
+    the adjoint row pass reads them (the profile gradient is a few KB)."""
     C = nxy // 2 + 1
     rs = min(rmax + 1, nxy // 2)
     rows = rs + 1
-    return {
+    half = C * (nxy // 2 + 1) * s
+    b = {
         'row_pass': rows * C * s + nr * s,
-        'column_pass': rows * C * s + C * (nxy // 2 + 1) * s,
+        'column_pass': rows * C * s + half,
         'sample': 5 * s * nvis + 4 * s * nvis,
     }
+    if grad:
+        b['sample'] += 4 * 8 * nvis
+        b['adjoint_column_pass'] = 2 * half + rows * C * s
+        b['adjoint_row_pass'] = rows * C * s + 2 * nr * 8
+    return b
 
 
 def main():
@@ -55,6 +64,8 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=12.0, help='CPU-baseline time budget (0: skip)')
     ap.add_argument('--nvis', type=int, default=1_000_000)
     ap.add_argument('--nxy', type=int, default=4096)
+    ap.add_argument('--copy-calibration-mb', type=int, default=0,
+                    help='first run 5 device-to-device copies of this many MiB (HBM byte-counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU form is a design "
@@ -71,6 +82,14 @@ def main():
     g = galario_amd.double
     dev = torch.device('cuda:0')
     torch.cuda.set_device(dev)
+    if args.copy_calibration_mb:
+        n = args.copy_calibration_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def gpu_model(wl):
         return g.sampleProfile(wl.intensity, wl.Rmin, wl.dR, wl.nxy, wl.dxy, wl.u, wl.v,
@@ -93,36 +112,46 @@ def main():
     assert abs(chi2_dev - chi2_host) <= 1e-12 * abs(chi2_host), (chi2_dev, chi2_host)
     assert not ta.take_oob_flag()
 
-    # timed region: K steps, per-stage HIP events recorded on the kernels' stream
-    ta.stage_timer(True)
-    stream = torch.cuda.current_stream(dev)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = e0.elapsed_time(e1) / args.steps
-    stage_ms, ncalls = ta.stage_times()
-    ta.stage_timer(False)
-    stage_avg = {k: v_ / max(ncalls, 1) for k, v_ in stage_ms.items()}
-
     import math
     rmax = min(int(math.ceil((wl.Rmin + len(wl.intensity) * wl.dR) / wl.dxy)), wl.nxy // 2)  # swept box half side
-    sb = stage_bytes(wl.nxy, len(wl.u), len(wl.intensity), rmax)
-    stages = {k: {'ms': round(stage_avg[k], 5), 'alg_bytes': sb[k],
-                  'GBps': round(sb[k] / (stage_avg[k] * 1e-3) / 1e9, 1) if stage_avg[k] > 0 else None}
-              for k in ta.STAGES}
-    dom = max(ta.STAGES, key=lambda k: stage_avg[k])
-    achieved = sb[dom] / (stage_avg[dom] * 1e-3) / 1e9
-    C = wl.nxy // 2 + 1
-    b_alg = sum(sb.values())                                                  # this algorithm's bytes
-    b_complex = 2 * 8 * wl.nxy * C + 13 * 8 * len(wl.u) + 8 * len(wl.intensity)   # complex-spectrum model
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn, steps, grad):
+        """K steps between two events on the kernels' stream; per-stage HIP events inside."""
+        ta.stage_timer(True)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        stage_ms, ncalls = ta.stage_times()
+        ta.stage_timer(False)
+        sb = stage_bytes(wl.nxy, len(wl.u), len(wl.intensity), rmax, grad=grad)
+        stages = {}
+        for k, b in sb.items():
+            avg = stage_ms[k] / max(ncalls, 1)
+            stages[k] = {'ms': round(avg, 5), 'alg_bytes': b, 'GBps': round(b / (avg * 1e-3) / 1e9, 1) if avg else None}
+        return ms, stages, sum(sb.values())
+
+    ms_per_step, stages, b_alg = timed(step, args.steps, grad=False)
     evals = 1000.0 / ms_per_step
+
+    def step_grad():
+        ds.chi2_profile_grad(I, wl.Rmin, wl.dR, wl.nxy, wl.dxy, check_central_pixel=False, **kw)
+
+    for _ in range(args.warmup):
+        step_grad()
+    ms_grad, stages_grad, b_alg_grad = timed(step_grad, args.steps, grad=True)
+    evals_grad = 1000.0 / ms_grad
+
+    dom = max(stages, key=lambda k: stages[k]['ms'])
+    achieved = stages[dom]['GBps']
+    C = wl.nxy // 2 + 1
+    b_complex = 2 * 8 * wl.nxy * C + 13 * 8 * len(wl.u) + 8 * len(wl.intensity)   # complex-spectrum model
 
     # host-pointer (drop-in) form, PCIe uploads included: a few calls
     nh = 5
@@ -164,8 +193,11 @@ def main():
                               'complex_spectrum_model_bytes': b_complex,
                               'complex_model_frac': round(b_complex * evals / (HBM_PEAK_GBS * 1e9), 4)},
         'stages': stages,
+        'forward_grad': {'evals_per_s': round(evals_grad, 2), 'ms_per_step': round(ms_grad, 5),
+                         'alg_bytes_per_eval': b_alg_grad,
+                         'frac': round(b_alg_grad * evals_grad / (HBM_PEAK_GBS * 1e9), 4),
+                         'stages': stages_grad},
         'host_api_evals_per_s_incl_pcie': round(host_rate, 2),
-        'wall_s_timed': round(wall, 3),
         'cpu_baseline': cpu,
     }
     print(json.dumps(res), flush=True)

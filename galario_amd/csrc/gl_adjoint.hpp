@@ -165,7 +165,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym(AdjColArgs<T
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
     constexpr int NPAIR = (C + 1) / 2;
-    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    __shared__ T lds[S::LDS_ELEMS];
     const int pair = xcd_column(blockIdx.x, gridDim.x);
     if (pair >= NPAIR) return;
     const int la = 2 * pair, lb = la + 1;
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjrow_sym(AdjRowArgs<T
     using S = FftShape<LOG2N>;
     using F = BlockFft<T, LOG2N, false>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
-    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    __shared__ T lds[S::LDS_ELEMS];
     __shared__ double red[16];
     const int tid = threadIdx.x;
     const int ma = 2 * blockIdx.x, mb = ma + 1;

@@ -3,8 +3,8 @@
 // One workgroup computes one complex N-point transform.  Each of the NT = N/E
 // threads keeps E = 16 elements (fewer for N < 1024) in registers; a pass of
 // radix R is a register-resident R-point DFT, and consecutive passes exchange
-// data through LDS (N complex + 1/16 padding, so the stride-R*16 B stores of
-// the early passes spread over all banks).  The first pass reads its inputs
+// data through LDS (N reals + 1/16 padding, real parts then imaginary parts,
+// so the stride-R stores of the early passes spread over all banks).  The first pass reads its inputs
 // straight from a caller functor and the last pass hands its outputs straight
 // to a caller functor, so both touch HBM coalesced (thread j owns elements
 // j + r*N/R) and the transform costs 2 (E>=16: log16 N - 1) LDS round trips.
@@ -166,31 +166,51 @@ template <typename T, int LOG2N, bool INV>
 struct BlockFft {
     using S = FftShape<LOG2N>;
     static constexpr int N = S::N, E = S::E, NT = S::NT;
+    static constexpr int TWC = 8;           // preloaded twiddle bases per pass and thread
+    using TwCache = cpx<T>[S::NPASS][TWC];
 
     static __device__ __forceinline__ cpx<T> twid(const cpx<T>* __restrict__ tw, int e) {
         const cpx<T> w = tw[e];
         return INV ? cconj(w) : w;
     }
 
-    // v[r] *= W_N^{r e1}, r = 1..R-1
+    // Table reads of pass P: for each of the thread's E/R butterflies the bases
+    // W^e1, W^2e1, W^4e1, W^8e1 (as many as log2 R).  They depend only on the
+    // thread index, so they are issued at kernel start, next to the data loads.
+    template <int P>
+    static __device__ __forceinline__ void preload(cpx<T> (&tc)[TWC], int tid, const cpx<T>* __restrict__ tw) {
+        constexpr int LR = S::log2radix(P);
+        constexpr int R = 1 << LR;
+        constexpr int NS = S::ns_before(P);
+        static_assert((E / R) * LR <= TWC, "twiddle cache too small");
+#pragma unroll
+        for (int b = 0; b < E / R; ++b) {
+            const int j = tid + b * NT;
+            const int e1 = (j & (NS - 1)) * (N / (NS * R));
+#pragma unroll
+            for (int q = 0; q < LR; ++q) tc[b * LR + q] = twid(tw, e1 << q);
+        }
+    }
+
+    // v[r] *= W_N^{r e1}, r = 1..R-1, from the bases W^e1, W^2e1, W^4e1, W^8e1
     template <int R>
-    static __device__ __forceinline__ void apply_twiddles(cpx<T>* v, int e1, const cpx<T>* __restrict__ tw) {
-        const cpx<T> t1 = twid(tw, e1);
+    static __device__ __forceinline__ void apply_twiddles(cpx<T>* v, const cpx<T>* base) {
+        const cpx<T> t1 = base[0];
         v[1] = cmul(v[1], t1);
         if constexpr (R >= 4) {
-            const cpx<T> t2 = twid(tw, 2 * e1);
+            const cpx<T> t2 = base[1];
             const cpx<T> t3 = cmul(t2, t1);
             v[2] = cmul(v[2], t2);
             v[3] = cmul(v[3], t3);
             if constexpr (R >= 8) {
-                const cpx<T> t4 = twid(tw, 4 * e1);
+                const cpx<T> t4 = base[2];
                 const cpx<T> t5 = cmul(t4, t1), t6 = cmul(t4, t2), t7 = cmul(t4, t3);
                 v[4] = cmul(v[4], t4);
                 v[5] = cmul(v[5], t5);
                 v[6] = cmul(v[6], t6);
                 v[7] = cmul(v[7], t7);
                 if constexpr (R >= 16) {
-                    const cpx<T> t8 = twid(tw, 8 * e1);
+                    const cpx<T> t8 = base[3];
                     v[8] = cmul(v[8], t8);
                     v[9] = cmul(v[9], cmul(t8, t1));
                     v[10] = cmul(v[10], cmul(t8, t2));
@@ -204,25 +224,23 @@ struct BlockFft {
         }
     }
 
-    // twiddle + DFT of pass p on the registers (E/R butterflies per thread)
+    // twiddle + DFT of pass P on the registers (E/R butterflies per thread)
     template <int P>
-    static __device__ __forceinline__ void compute(cpx<T> (&v)[E], int tid, const cpx<T>* __restrict__ tw) {
-        constexpr int R = 1 << S::log2radix(P);
+    static __device__ __forceinline__ void compute(cpx<T> (&v)[E], const cpx<T> (&tc)[TWC]) {
+        constexpr int LR = S::log2radix(P);
+        constexpr int R = 1 << LR;
         constexpr int NS = S::ns_before(P);
 #pragma unroll
         for (int b = 0; b < E / R; ++b) {
-            if constexpr (NS > 1) {
-                const int j = tid + b * NT;
-                const int k = j & (NS - 1);
-                apply_twiddles<R>(&v[b * R], k * (N / (NS * R)), tw);
-            }
+            if constexpr (NS > 1) apply_twiddles<R>(&v[b * R], &tc[b * LR]);
             dft<R, INV>(&v[b * R]);
         }
     }
 
-    // results of pass P -> LDS at their Stockham output positions
-    template <int P>
-    static __device__ __forceinline__ void store_lds(cpx<T>* lds, const cpx<T> (&v)[E], int tid) {
+    // one component (0: real, 1: imaginary) of the results of pass P -> LDS
+    // at their Stockham output positions
+    template <int P, int PART>
+    static __device__ __forceinline__ void store_lds(T* lds, const cpx<T> (&v)[E], int tid) {
         constexpr int R = 1 << S::log2radix(P);
         constexpr int NS = S::ns_before(P);
 #pragma unroll
@@ -231,46 +249,60 @@ struct BlockFft {
             const int k = j & (NS - 1);
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = v[b * R + r];
+            for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = PART ? v[b * R + r].im : v[b * R + r].re;
         }
     }
 
-    // inputs of pass P <- LDS
-    template <int P>
-    static __device__ __forceinline__ void load_lds(cpx<T> (&v)[E], const cpx<T>* lds, int tid) {
+    // one component of the inputs of pass P <- LDS
+    template <int P, int PART>
+    static __device__ __forceinline__ void load_lds(cpx<T> (&v)[E], const T* lds, int tid) {
         constexpr int R = 1 << S::log2radix(P);
 #pragma unroll
         for (int b = 0; b < E / R; ++b) {
             const int j = tid + b * NT;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b * R + r] = lds[lpad(j + r * (N / R))];
+            for (int r = 0; r < R; ++r) {
+                const T x = lds[lpad(j + r * (N / R))];
+                if (PART) v[b * R + r].im = x;
+                else v[b * R + r].re = x;
+            }
         }
     }
 
     // Passes 1..NPASS-1 after the first pass has been computed in registers:
-    // exchange through LDS, twiddle, DFT.  On return thread tid holds
-    // X[tid + r*N/E] in v[r] (the last pass is radix E with one butterfly per thread).
+    // exchange through LDS (real parts, then imaginary parts, so the buffer
+    // holds N reals), twiddle, DFT.  On return thread tid holds X[tid + r*N/E]
+    // in v[r] (the last pass is radix E with one butterfly per thread).
     template <int P = 1>
-    static __device__ __forceinline__ void rest(cpx<T> (&v)[E], cpx<T>* lds, int tid, const cpx<T>* __restrict__ tw) {
+    static __device__ __forceinline__ void rest(cpx<T> (&v)[E], T* lds, int tid, const TwCache& tc) {
         if constexpr (P < S::NPASS) {
-            store_lds<P - 1>(lds, v, tid);
+            store_lds<P - 1, 0>(lds, v, tid);
             __syncthreads();
-            load_lds<P>(v, lds, tid);
+            load_lds<P, 0>(v, lds, tid);
             __syncthreads();
-            compute<P>(v, tid, tw);
-            rest<P + 1>(v, lds, tid, tw);
+            store_lds<P - 1, 1>(lds, v, tid);
+            __syncthreads();
+            load_lds<P, 1>(v, lds, tid);
+            __syncthreads();
+            compute<P>(v, tc[P]);
+            rest<P + 1>(v, lds, tid, tc);
         }
     }
 
     // Full transform.  load(n) returns input element n; it is called only for
-    // n in [0, Rs) U [N-Rs, N) (zeros elsewhere).  On return v[r] holds
-    // X[tid + r*N/E].
+    // n in [0, Rs) U [N-Rs, N) (zeros elsewhere).  lds: LDS_ELEMS reals.  On
+    // return v[r] holds X[tid + r*N/E], and every thread has finished reading LDS.
     template <typename Loader>
-    static __device__ __forceinline__ void run(cpx<T> (&v)[E], cpx<T>* lds, int tid, int Rs,
+    static __device__ __forceinline__ void run(cpx<T> (&v)[E], T* lds, int tid, int Rs,
                                                const cpx<T>* __restrict__ tw, Loader&& load) {
         constexpr int R0 = E;            // first pass is radix E, one butterfly per thread
         constexpr int STRIDE = N / R0;   // == NT
         const int j = tid;
+        TwCache tc;
+        static_for<1, S::NPASS>([&](auto pc) {
+            constexpr int P = decltype(pc)::value;
+            preload<P>(tc[P], tid, tw);
+        });
         if (Rs <= STRIDE) {
             // at most x[j] (r = 0) and x[j + (R0-1) N/R0] (r = R0-1) are non-zero
             const cpx<T> z = {T(0), T(0)};
@@ -286,7 +318,7 @@ struct BlockFft {
             }
             dft<R0, INV>(v);
         }
-        rest<1>(v, lds, tid, tw);
+        rest<1>(v, lds, tid, tc);
     }
 };
 

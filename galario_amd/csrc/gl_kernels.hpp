@@ -129,7 +129,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass(RowArgs<T> a) {
     using S = FftShape<LOG2N>;
     using F = BlockFft<T, LOG2N, false>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
-    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    __shared__ T lds[S::LDS_ELEMS];
     const int tid = threadIdx.x;
     const int kk_a = 2 * blockIdx.x, kk_b = kk_a + 1;
     const int ka = compact_to_row(kk_a, a.Rs, N), kb = compact_to_row(kk_b, a.Rs, N);
@@ -137,17 +137,26 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass(RowArgs<T> a) {
     F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
         return cpx<T>{row_source<T, SRC>(a, ka, n), row_source<T, SRC>(a, kb, n)};
     });
-#pragma unroll
-    for (int r = 0; r < E; ++r) lds[lpad(tid + r * NT)] = v[r];
-    __syncthreads();
+    // X_a[l] = (Z[l] + conj Z[N-l]) / 2, X_b[l] = (Z[l] - conj Z[N-l]) / 2i, one component at a time
     const int C = N / 2 + 1;
     cpx<T>* outa = a.Tm + size_t(kk_a) * a.ldT;
     cpx<T>* outb = a.Tm + size_t(kk_b) * a.ldT;
+#pragma unroll
+    for (int r = 0; r < E; ++r) lds[lpad(tid + r * NT)] = v[r].re;
+    __syncthreads();
     for (int l = tid; l < C; l += NT) {
-        const cpx<T> za = lds[lpad(l)];
-        const cpx<T> zb = lds[lpad((N - l) & (N - 1))];
-        outa[l] = cpx<T>{T(0.5) * (za.re + zb.re), T(0.5) * (za.im - zb.im)};
-        outb[l] = cpx<T>{T(0.5) * (za.im + zb.im), T(0.5) * (zb.re - za.re)};
+        const T zr = lds[lpad(l)], wr = lds[lpad((N - l) & (N - 1))];
+        outa[l].re = T(0.5) * (zr + wr);
+        outb[l].im = T(0.5) * (wr - zr);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; ++r) lds[lpad(tid + r * NT)] = v[r].im;
+    __syncthreads();
+    for (int l = tid; l < C; l += NT) {
+        const T zi = lds[lpad(l)], wi = lds[lpad((N - l) & (N - 1))];
+        outa[l].im = T(0.5) * (zi - wi);
+        outb[l].re = T(0.5) * (zi + wi);
     }
 }
 
@@ -176,7 +185,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass(ColArgs<T> a) {
     using F = BlockFft<T, LOG2N, false>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
-    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    __shared__ T lds[S::LDS_ELEMS];
     const int col = xcd_column(blockIdx.x, gridDim.x);
     if (col >= C) return;
     const int tid = threadIdx.x;
@@ -302,7 +311,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass_sym(SymRowArgs<
     using F = BlockFft<T, LOG2N, false>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
-    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    __shared__ T lds[S::LDS_ELEMS];
     const int tid = threadIdx.x;
     const int ma = 2 * blockIdx.x, mb = ma + 1;
     const bool hb = mb < a.nrows;
@@ -345,7 +354,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass_sym(SymColArgs<
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
     constexpr int NPAIR = (C + 1) / 2;
-    __shared__ cpx<T> lds[S::LDS_ELEMS];
+    __shared__ T lds[S::LDS_ELEMS];
     const int pair = xcd_column(blockIdx.x, gridDim.x);
     if (pair >= NPAIR) return;
     const int la = 2 * pair, lb = la + 1;
