@@ -944,11 +944,13 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         const SymGeom g = sym_geom(L, nxy, ps.rmax);
         // Hh is kept zero between calls (the adjoint column pass clears what it read); it is
         // cleared here only when new or after a call that did not reach that pass
-        const size_t nHh = size_t(g.C + 1) * g.ldH;
-        if (w.Hh.cap < nHh * sizeof(double)) w.Hh_zero = false;
-        double* Hh = w.Hh.get<double>(nHh);
+        // (followed by the replicas of the hot tile, which that pass folds in and clears)
+        const size_t nHh = size_t(g.C + 1) * g.ldH, nAll = nHh + GHOT_ELEMS;
+        if (w.Hh.cap < nAll * sizeof(double)) w.Hh_zero = false;
+        double* Hh = w.Hh.get<double>(nAll);
+        double* hot = Hh + nHh;
         if (!w.Hh_zero) {
-            hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(nHh, 256)), dim3(256), 0, s, Hh, nHh);
+            hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(nAll, 256)), dim3(256), 0, s, Hh, nAll);
             HC(hipGetLastError());
             w.Hh_zero = true;
         }
@@ -965,7 +967,7 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         sa.flag = w.flag_dev.get<unsigned>(1);
         const unsigned nb = blocks_for(size_t(nd), 256);
         double* part = w.part.get<double>(size_t(4) * (nb ? nb : 1));
-        GradArgs<T> ga{Hh, part};
+        GradArgs<T> ga{Hh, hot, part};
         t_timer.mark(ST_SAMPLE, s);
         w.Hh_zero = false;
         if (nb) hipLaunchKernelGGL((k_sample_grad_sym<T>), dim3(nb), dim3(256), 0, s, sa, ga);
@@ -976,7 +978,7 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         const int ldTh = (g.C + 1 + 7) & ~7;
         T* Th = w.Th.get<T>(size_t(g.nrows) * ldTh);
         const cpx<T>* tw = twiddles<T>(nxy);
-        AdjColArgs<T> aca{Hh, g.ldH, nxy, g.Rs, tw, Th, ldTh};
+        AdjColArgs<T> aca{Hh, hot, g.ldH, nxy, g.Rs, tw, Th, ldTh};
         t_timer.mark(ST_ADJCOL, s);
         launch_adjcol_sym<T>(L, 8 * ((g.pairs_c + 7) / 8), s, aca);
         HC(hipGetLastError());

@@ -25,6 +25,8 @@
 // (x/cos inc)^2 tan(inc) / r; the central pixel is linear in I[0..iIN+1].
 #pragma once
 
+#include <climits>
+
 #include "gl_kernels.hpp"
 
 namespace gl {
@@ -34,29 +36,72 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtom
 // ---------------------------------------------------------------------------
 // sample + its adjoint: chi2 and (PA, dRA, dDec) partials per workgroup,
 // corner gradients scattered into Hh.
+//
+// Global FP64 atomics on gfx950 are resolved beyond the per-XCD L2, so many
+// atomics on one address serialise: with log-uniform baselines ~10% of all
+// points fall in the few cells around the origin.  Each workgroup therefore
+// first accumulates its corner gradients in LDS:
+//   * cells with column < GHOT and folded row < GHOT go to a GHOT x GHOT LDS
+//     tile, flushed into one of GREP replicas of that tile (by workgroup
+//     index); the adjoint column pass folds the replicas into Hh;
+//   * other cells go to an LDS window over the bounding box of the
+//     workgroup's cells when it holds at most GWIN cells (points are stored in
+//     a locality order, so it usually does), flushed with one atomic per
+//     non-zero cell; otherwise straight to Hh.
 // ---------------------------------------------------------------------------
+constexpr int GHOT = 16;          // hot tile: columns [0, GHOT) x folded rows [0, GHOT)
+constexpr int GREP = 32;          // replicas of the hot tile in global memory
+constexpr int GWIN = 4096;        // LDS window capacity (cells)
+constexpr int GHOT_ELEMS = GREP * GHOT * GHOT;
+
 template <typename T>
 struct GradArgs {
     double* Hh;          // adjoint of H, same layout (column stride ldH), zero on entry
+    double* hot;         // GREP x GHOT x GHOT replicas [rep][col][row], zero on entry
     double* partials;    // 4 x nblocks: chi2, dPA, ddRA, ddDec
 };
 
+__device__ __forceinline__ void lds_add_f64(double* p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ int wave_min(int x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = min(x, __shfl_xor(x, off, 64));
+    return x;
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = max(x, __shfl_xor(x, off, 64));
+    return x;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_sample_grad_sym(SampleArgs<T> a, GradArgs<T> g) {
-    __shared__ double red[4];
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    __shared__ double win[GWIN];
+    __shared__ double hot[GHOT * GHOT];
+    __shared__ double red[4][4];
+    __shared__ int box[4][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int i = blockIdx.x * 256 + tid;
     double c2 = 0.0, gpa = 0.0, gra = 0.0, gde = 0.0;
-    bool oob = false;
+    double gk[4] = {0.0, 0.0, 0.0, 0.0};         // corners (fl_u, m0), (fl_u, m1), (fu1, m0), (fu1, m1)
+    int cu0 = 0, cu1 = 0, cm0 = 0, cm1 = 0;
+    bool live = false, oob = false;
+    for (int j = tid; j < GHOT * GHOT; j += 256) hot[j] = 0.0;
     if (i < a.nd) {
         const T u0 = a.u[i], v0 = a.v[i];
         const GridPos<T> p = grid_pos(a, u0, v0);
         oob = p.oob;
         const T q = p.q, t = p.t;
         const int half = a.nrow / 2;
-        const int m0 = abs(p.fl_v - half), m1 = abs(p.fv1 - half);
-        const size_t i00 = size_t(p.fl_u) * a.ldH + m0, i01 = size_t(p.fl_u) * a.ldH + m1;
-        const size_t i10 = size_t(p.fu1) * a.ldH + m0, i11 = size_t(p.fu1) * a.ldH + m1;
-        const T y0 = a.H[i00], y1 = a.H[i01], y3 = a.H[i10], y2 = a.H[i11];
+        cu0 = p.fl_u;
+        cu1 = p.fu1;
+        cm0 = abs(p.fl_v - half);
+        cm1 = abs(p.fv1 - half);
+        const T y0 = a.H[size_t(cu0) * a.ldH + cm0], y1 = a.H[size_t(cu0) * a.ldH + cm1];
+        const T y3 = a.H[size_t(cu1) * a.ldH + cm0], y2 = a.H[size_t(cu1) * a.ldH + cm1];
         const T z = (t * (q * ((y0 + y2) - (y1 + y3))) + (t * (y1 - y0) + q * (y3 - y0))) + y0;
         const T a0 = fabs(y0), a1 = fabs(y1), a2 = fabs(y2), a3 = fabs(y3);
         T amp = a0;
@@ -89,34 +134,78 @@ __global__ __launch_bounds__(256) void k_sample_grad_sym(SampleArgs<T> a, GradAr
         gra = gth * two_pi * double(u0);
         gde = gth * two_pi * double(v0);
         if (!oob) {
-            const double b0 = double(1 - t) * double(1 - q), b1 = double(t) * double(1 - q);
-            const double b2 = double(t) * double(q), b3 = double(1 - t) * double(q);
             const auto sgn = [](T y) { return double((y > T(0)) - (y < T(0))); };
-            const double g0 = gA * b0 * sgn(y0), g1 = gA * b1 * sgn(y1);
-            const double g2 = gA * b2 * sgn(y2), g3 = gA * b3 * sgn(y3);
-            if (g0 != 0.0) atomic_add_f64(g.Hh + i00, g0);
-            if (g1 != 0.0) atomic_add_f64(g.Hh + i01, g1);
-            if (g3 != 0.0) atomic_add_f64(g.Hh + i10, g3);
-            if (g2 != 0.0) atomic_add_f64(g.Hh + i11, g2);
+            gk[0] = gA * (double(1 - t) * double(1 - q)) * sgn(y0);
+            gk[1] = gA * (double(t) * double(1 - q)) * sgn(y1);
+            gk[2] = gA * (double(1 - t) * double(q)) * sgn(y3);
+            gk[3] = gA * (double(t) * double(q)) * sgn(y2);
+            live = true;
         } else {
             c2 = gpa = gra = gde = double(NAN);
         }
     }
     if (oob && a.flag) atomicOr(a.flag, 1u);
-    const unsigned nb = gridDim.x;
-    double* P = g.partials;
-    double v;
-    v = block_sum256(c2, red);
-    if (threadIdx.x == 0) P[blockIdx.x] = v;
+    // bounding box of the workgroup's cells
+    int b0 = live ? cu0 : INT_MAX, b1 = live ? cu1 : -1;
+    int b2 = live ? min(cm0, cm1) : INT_MAX, b3 = live ? max(cm0, cm1) : -1;
+    b0 = wave_min(b0);
+    b1 = wave_max(b1);
+    b2 = wave_min(b2);
+    b3 = wave_max(b3);
+    if (lane == 0) {
+        box[wv][0] = b0;
+        box[wv][1] = b1;
+        box[wv][2] = b2;
+        box[wv][3] = b3;
+    }
     __syncthreads();
-    v = block_sum256(gpa, red);
-    if (threadIdx.x == 0) P[nb + blockIdx.x] = v;
+    const int umin = min(min(box[0][0], box[1][0]), min(box[2][0], box[3][0]));
+    const int umax = max(max(box[0][1], box[1][1]), max(box[2][1], box[3][1]));
+    const int mmin = min(min(box[0][2], box[1][2]), min(box[2][2], box[3][2]));
+    const int mmax = max(max(box[0][3], box[1][3]), max(box[2][3], box[3][3]));
+    const int wu = umax - umin + 1, wm = mmax - mmin + 1;
+    const bool use_win = umax >= 0 && wu * wm <= GWIN;     // (wu, wm <= 2^13, no overflow)
+    const int nwin = use_win ? wu * wm : 0;
+    for (int j = tid; j < nwin; j += 256) win[j] = 0.0;
     __syncthreads();
-    v = block_sum256(gra, red);
-    if (threadIdx.x == 0) P[2 * nb + blockIdx.x] = v;
+    if (live) {
+        const int cu[4] = {cu0, cu0, cu1, cu1}, cm[4] = {cm0, cm1, cm0, cm1};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (gk[k] == 0.0) continue;
+            if (cu[k] < GHOT && cm[k] < GHOT) lds_add_f64(&hot[cu[k] * GHOT + cm[k]], gk[k]);
+            else if (use_win) lds_add_f64(&win[(cu[k] - umin) * wm + (cm[k] - mmin)], gk[k]);
+            else atomic_add_f64(g.Hh + size_t(cu[k]) * a.ldH + cm[k], gk[k]);
+        }
+    }
     __syncthreads();
-    v = block_sum256(gde, red);
-    if (threadIdx.x == 0) P[3 * nb + blockIdx.x] = v;
+    {
+        double* rep = g.hot + size_t(blockIdx.x % GREP) * (GHOT * GHOT);
+        for (int j = tid; j < GHOT * GHOT; j += 256) {
+            const double x = hot[j];
+            if (x != 0.0) atomic_add_f64(rep + j, x);
+        }
+        for (int j = tid; j < nwin; j += 256) {
+            const double x = win[j];
+            if (x != 0.0) {
+                const int c = j / wm;
+                atomic_add_f64(g.Hh + size_t(umin + c) * a.ldH + (mmin + (j - c * wm)), x);
+            }
+        }
+    }
+    // per-workgroup partials of chi2, dPA, ddRA, ddDec (fixed order)
+    c2 = wave_sum(c2);
+    gpa = wave_sum(gpa);
+    gra = wave_sum(gra);
+    gde = wave_sum(gde);
+    if (lane == 0) {
+        red[0][wv] = c2;
+        red[1][wv] = gpa;
+        red[2][wv] = gra;
+        red[3][wv] = gde;
+    }
+    __syncthreads();
+    if (tid < 4) g.partials[size_t(tid) * gridDim.x + blockIdx.x] = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
 }
 
 // Deterministic sums of 4 streams of n partials: *dst[j] = sum_i p[j*n + i].
@@ -148,6 +237,7 @@ __global__ __launch_bounds__(1024) void k_sum_partials4(const double* __restrict
 template <typename T>
 struct AdjColArgs {
     double* Hh;
+    double* hot;         // GREP replicas of the hot tile, added to Hh columns < GHOT, rows < GHOT
     int ldH;
     int N, Rs;           // Rs': outputs m = 0..Rs'
     const cpx<T>* tw;
@@ -173,17 +263,34 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym(AdjColArgs<T
     const int tid = threadIdx.x;
     double* ha = a.Hh + size_t(la) * a.ldH;
     double* hb = ha + a.ldH;
+    const bool hot_cols = la < GHOT;   // GHOT is even: lb < GHOT too
+    const auto hot_sum = [&](int l, int p) {
+        double s = 0.0;
+        for (int r = 0; r < GREP; ++r) s += a.hot[(r * GHOT + l) * GHOT + p];
+        return s;
+    };
     cpx<T> v[E];
     F::run(v, lds, tid, N / 2, a.tw, [&](int k) -> cpx<T> {
         const int p = k <= N / 2 ? k : N - k;
         const double wk = ext_weight(k, N);
-        return cpx<T>{T(wk * ha[p]), has_b ? T(wk * hb[p]) : T(0)};
+        double xa = ha[p], xb = has_b ? hb[p] : 0.0;
+        if (hot_cols && p < GHOT) {
+            xa += hot_sum(la, p);
+            if (has_b) xb += hot_sum(lb, p);
+        }
+        return cpx<T>{T(wk * xa), T(wk * xb)};
     });
     // all reads of Hh are done (the FFT synchronised after its first pass): clear it for the next call
     for (int p = tid; p <= N / 2; p += NT) {
         ha[p] = 0.0;
         if (has_b) hb[p] = 0.0;
     }
+    if (hot_cols)
+        for (int j = tid; j < GREP * GHOT; j += NT) {
+            const int r = j / GHOT, p = j % GHOT;
+            a.hot[(r * GHOT + la) * GHOT + p] = 0.0;
+            if (has_b) a.hot[(r * GHOT + lb) * GHOT + p] = 0.0;
+        }
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         const int m = tid + r * NT;
