@@ -8,9 +8,13 @@ A step is one chi2Profile evaluation with all inputs resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
 
-Prints one JSON line.  `roofline` is measured live: HIP events around each
-stage on the stream the kernels run on give each kernel's average duration;
-`achieved` = that kernel's algorithmic bytes / duration.  `cpu_baseline`
+Prints one JSON line.  `value` is timed with HIP events on the kernels' stream
+around K calls (no per-stage events in that loop).  `roofline` is measured
+live: each forward kernel is launched 20 times back to back inside a pair of
+HIP events (galario_stage_repeat), giving its average launch duration;
+`achieved` = that kernel's algorithmic bytes / duration for the slowest one.
+`traffic` is that kernel's HBM bytes per launch from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE summary (calibrated on a copy kernel).  `cpu_baseline`
 times the C/OpenMP restatement of galario's CPU loops (FFT by scipy.fft) on a
 bounded sample of the same workload.
 """
@@ -54,6 +58,24 @@ def stage_bytes(nxy, nvis, nr, rmax, grad=False, s=8):
         b['adjoint_column_pass'] = 2 * half + rows * C * s
         b['adjoint_row_pass'] = rows * C * s + 2 * nr * 8
     return b
+
+
+KERNEL_NAMES = {'row_pass': 'k_rowpass_sym<double, 12>', 'column_pass': 'k_colpass_sym<double, 12>',
+                'sample': 'k_sample<double, 1, true>'}
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hbm_traffic_profile_nxy4096_nvis1e6_f64.txt')
+
+
+def measured_traffic(kernel, nxy, nvis):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (lines 'traffic <kernel> <bytes>'), for the default workload only; None otherwise."""
+    if (nxy, nvis) != (4096, 1_000_000) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.rstrip('\n').split('\t')
+            if len(parts) == 3 and parts[0] == 'traffic' and parts[1] == kernel:
+                return int(parts[2])
+    return None
 
 
 def main():
@@ -116,40 +138,74 @@ def main():
     rmax = min(int(math.ceil((wl.Rmin + len(wl.intensity) * wl.dR) / wl.dxy)), wl.nxy // 2)  # swept box half side
     stream = torch.cuda.current_stream(dev)
 
-    def timed(fn, steps, grad):
-        """K steps between two events on the kernels' stream; per-stage HIP events inside."""
-        ta.stage_timer(True)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
+    def run_timed(fn, steps, blocks=10):
+        """`steps` calls between events on the kernels' stream, in `blocks` equal blocks:
+        (mean ms per call over all steps, std and min over the blocks' per-call means)."""
+        per = max(1, steps // blocks)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(blocks + 1)]
         torch.cuda.synchronize()
-        e0.record(stream)
+        ev[0].record(stream)
+        for b in range(blocks):
+            for _ in range(per):
+                fn()
+            ev[b + 1].record(stream)
+        torch.cuda.synchronize()
+        blk = np.array([ev[b].elapsed_time(ev[b + 1]) / per for b in range(blocks)])
+        return float(ev[0].elapsed_time(ev[blocks]) / (per * blocks)), float(blk.std()), float(blk.min()), per * blocks
+
+    def stage_breakdown(fn, steps, grad):
+        """Per-stage HIP events inside each call (a separate pass: the events add their own gaps)."""
+        ta.stage_timer(True)
         for _ in range(steps):
             fn()
-        e1.record(stream)
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / steps
         stage_ms, ncalls = ta.stage_times()
         ta.stage_timer(False)
         sb = stage_bytes(wl.nxy, len(wl.u), len(wl.intensity), rmax, grad=grad)
         stages = {}
-        for k, b in sb.items():
+        for k in ta.STAGES:
             avg = stage_ms[k] / max(ncalls, 1)
-            stages[k] = {'ms': round(avg, 5), 'alg_bytes': b, 'GBps': round(b / (avg * 1e-3) / 1e9, 1) if avg else None}
-        return ms, stages, sum(sb.values())
+            if avg == 0.0:
+                continue
+            b = sb.get(k)
+            stages[k] = {'ms': round(avg, 5), 'alg_bytes': b,
+                         'GBps': round(b / (avg * 1e-3) / 1e9, 1) if b else None}
+        return stages, sum(sb.values())
 
-    ms_per_step, stages, b_alg = timed(step, args.steps, grad=False)
+    def kernel_launch_ms(fn, stage, reps=20, calls=5):
+        """Average launch duration of one forward kernel: `reps` back-to-back launches inside the
+        stage's event interval, `calls` calls."""
+        ta.stage_timer(True)
+        ta.stage_repeat(stage, reps)
+        for _ in range(calls):
+            fn()
+        torch.cuda.synchronize()
+        stage_ms, ncalls = ta.stage_times()
+        ta.stage_repeat(None, 1)
+        ta.stage_timer(False)
+        return stage_ms[stage] / (ncalls * reps)
+
+    ms_per_step, ms_std, ms_min, nsteps = run_timed(step, args.steps)
     evals = 1000.0 / ms_per_step
+    stages, b_alg = stage_breakdown(step, min(args.steps, 50), grad=False)
+    kernels = {}
+    sb = stage_bytes(wl.nxy, len(wl.u), len(wl.intensity), rmax)
+    for k in ('row_pass', 'column_pass', 'sample'):
+        kms = kernel_launch_ms(step, k)
+        kernels[k] = {'avg_launch_ms': round(kms, 5), 'alg_bytes': sb[k], 'GBps': round(sb[k] / (kms * 1e-3) / 1e9, 1)}
 
     def step_grad():
         ds.chi2_profile_grad(I, wl.Rmin, wl.dR, wl.nxy, wl.dxy, check_central_pixel=False, **kw)
 
     for _ in range(args.warmup):
         step_grad()
-    ms_grad, stages_grad, b_alg_grad = timed(step_grad, args.steps, grad=True)
+    ms_grad, ms_grad_std, ms_grad_min, _ = run_timed(step_grad, args.steps)
     evals_grad = 1000.0 / ms_grad
+    stages_grad, b_alg_grad = stage_breakdown(step_grad, min(args.steps, 50), grad=True)
 
-    dom = max(stages, key=lambda k: stages[k]['ms'])
-    achieved = stages[dom]['GBps']
+    dom = max(kernels, key=lambda k: kernels[k]['avg_launch_ms'])
+    achieved = kernels[dom]['GBps']
+    traffic = measured_traffic(KERNEL_NAMES[dom], wl.nxy, len(wl.u))
     C = wl.nxy // 2 + 1
     b_complex = 2 * 8 * wl.nxy * C + 13 * 8 * len(wl.u) + 8 * len(wl.intensity)   # complex-spectrum model
 
@@ -179,21 +235,26 @@ def main():
                          f'{abs(c - chi2_host) / abs(chi2_host):.1e}'}
 
     res = {
-        'metric': METRIC, 'value': round(evals, 2), 'unit': 'evals/s', 'n_gpus': 1, 'steps': args.steps,
+        'metric': METRIC, 'value': round(evals, 2), 'unit': 'evals/s', 'n_gpus': 1, 'steps': nsteps,
         'warmup': args.warmup, 'ms_per_step': round(ms_per_step, 5), 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
         'config': {'workload': f'chi2Profile forward, plain device-resident form (dataset uploaded once in '
                                f'locality order), nxy={wl.nxy}, Nvis={len(wl.u)}, Nr={len(wl.intensity)}, FP64',
                    'nxy': wl.nxy, 'nvis': len(wl.u), 'nr': len(wl.intensity), 'rmax': rmax,
                    'parallelism': 'single GPU'},
+        'ms_per_step_std': round(ms_std, 5), 'ms_per_step_min': round(ms_min, 5),
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None},
+                     'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
+                     'alg_bytes_per_launch': kernels[dom]['alg_bytes'],
+                     'avg_launch_ms': kernels[dom]['avg_launch_ms']},
+        'kernels': kernels,
         'pipeline_roofline': {'alg_bytes_per_eval': b_alg,
                               'frac': round(b_alg * evals / (HBM_PEAK_GBS * 1e9), 4),
                               'complex_spectrum_model_bytes': b_complex,
                               'complex_model_frac': round(b_complex * evals / (HBM_PEAK_GBS * 1e9), 4)},
         'stages': stages,
         'forward_grad': {'evals_per_s': round(evals_grad, 2), 'ms_per_step': round(ms_grad, 5),
+                         'ms_per_step_std': round(ms_grad_std, 5), 'ms_per_step_min': round(ms_grad_min, 5),
                          'alg_bytes_per_eval': b_alg_grad,
                          'frac': round(b_alg_grad * evals_grad / (HBM_PEAK_GBS * 1e9), 4),
                          'stages': stages_grad},

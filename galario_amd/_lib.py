@@ -56,6 +56,7 @@ def _load():
     lib.galario_free.restype = None
     lib.galario_take_oob_flag.argtypes = [_ip]
     lib.galario_stage_timer.argtypes = [_c_int]
+    lib.galario_stage_repeat.argtypes = [_c_int, _c_int]
     lib.galario_stage_times.argtypes = [ctypes.POINTER(ctypes.c_double), _c_int, _ip]
     for suf, real in (('f64', ctypes.c_double), ('f32', ctypes.c_float)):
         R = real

@@ -139,9 +139,11 @@ thread_local std::map<int, Workspace*> t_ws;
 // When enabled, the _dev entry points record a HIP event before each stage and
 // after the last one, on the stream the kernels run on; galario_stage_times
 // synchronises and returns the summed durations per stage.
-enum { ST_ROW = 0, ST_COL = 1, ST_SAMPLE = 2, ST_ADJCOL = 3, ST_ADJROW = 4, NSTAGE = 5 };
+enum { ST_ROW = 0, ST_COL = 1, ST_SAMPLE = 2, ST_ADJCOL = 3, ST_ADJROW = 4, ST_REDUCE = 5, NSTAGE = 6 };
 struct StageTimer {
     bool on = false;
+    int rep_stage = -1, rep_n = 1;           // while on: launch that stage's kernel rep_n times
+    int reps(int stage) const { return (on && stage == rep_stage) ? rep_n : 1; }
     std::vector<hipEvent_t> pool;            // free events
     std::vector<std::vector<std::pair<int, hipEvent_t>>> calls;   // per call: (stage, event) in order
     std::vector<std::pair<int, hipEvent_t>> cur;
@@ -158,7 +160,7 @@ struct StageTimer {
         cur.emplace_back(stage, e);
     }
     void end_call(hipStream_t s) {
-        if (!on) return;
+        if (!on || cur.empty()) return;
         mark(NSTAGE, s);
         calls.push_back(std::move(cur));
         cur.clear();
@@ -415,9 +417,9 @@ bool build_spectrum_sym(Workspace& w, hipStream_t s, const SweepPrm<T>& sw, int 
     SymRowArgs<T> ra{sw, N, g.Rs, g.nrows, g.ldT, tw, Ts, zero, nzero, zero1};
     SymColArgs<T> ca{Ts, N, g.Rs, g.ldT, tw, H, g.ldH};
     t_timer.mark(ST_ROW, s);
-    launch_rowpass_sym<T>(L, g.pairs_r, s, ra);
+    for (int r = t_timer.reps(ST_ROW); r > 0; --r) launch_rowpass_sym<T>(L, g.pairs_r, s, ra);
     t_timer.mark(ST_COL, s);
-    launch_colpass_sym<T>(L, 8 * ((g.pairs_c + 7) / 8), s, ca);
+    for (int r = t_timer.reps(ST_COL); r > 0; --r) launch_colpass_sym<T>(L, 8 * ((g.pairs_c + 7) / 8), s, ca);
     HC(hipGetLastError());
     *H_out = H;
     *ldH_out = g.ldH;
@@ -534,8 +536,11 @@ void launch_chi2(Workspace& w, hipStream_t s, SampleArgs<T> a, const T* re, cons
     a.w = wt;
     a.partials = w.part.get<double>(nb ? nb : 1);
     t_timer.mark(ST_SAMPLE, s);
-    if (nb && a.H) hipLaunchKernelGGL((k_sample<T, MODE_CHI2, true>), dim3(nb), dim3(256), 0, s, a);
-    else if (nb) hipLaunchKernelGGL((k_sample<T, MODE_CHI2, false>), dim3(nb), dim3(256), 0, s, a);
+    for (int r = t_timer.reps(ST_SAMPLE); r > 0; --r) {
+        if (nb && a.H) hipLaunchKernelGGL((k_sample<T, MODE_CHI2, true>), dim3(nb), dim3(256), 0, s, a);
+        else if (nb) hipLaunchKernelGGL((k_sample<T, MODE_CHI2, false>), dim3(nb), dim3(256), 0, s, a);
+    }
+    t_timer.mark(ST_REDUCE, s);
     hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, a.partials, int(nb), chi2_d, chi2_t);
     HC(hipGetLastError());
 }
@@ -587,6 +592,7 @@ void download(X* h, const X* d, size_t n, hipStream_t s) {
 
 // Synchronise the own stream and raise if a point fell outside the grid.
 void finish_host_call(Workspace& w, hipStream_t s, const unsigned* flag_d) {
+    t_timer.end_call(s);
     unsigned* hf = reinterpret_cast<unsigned*>(w.h_scal + 1);
     HC(hipMemcpyAsync(hf, flag_d, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HC(hipStreamSynchronize(s));
@@ -919,6 +925,7 @@ int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, in
         sa.flag = w.flag_dev.get<unsigned>(1);
         sa.perm = perm;
         launch_vis(s, sa, static_cast<cpx<T>*>(vis_d));
+        t_timer.end_call(s);
     });
 }
 
@@ -971,6 +978,7 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         t_timer.mark(ST_SAMPLE, s);
         w.Hh_zero = false;
         if (nb) hipLaunchKernelGGL((k_sample_grad_sym<T>), dim3(nb), dim3(256), 0, s, sa, ga);
+        t_timer.mark(ST_REDUCE, s);
         Dst4 dst{{chi2_d, grad_geom + 1, grad_geom + 2, grad_geom + 3}};
         hipLaunchKernelGGL(k_sum_partials4, dim3(1), dim3(1024), 0, s, part, int(nb), dst);
         HC(hipGetLastError());
@@ -1125,6 +1133,16 @@ int galario_stage_timer(int enable) {
         t_timer.calls.clear();
         for (auto& pe : t_timer.cur) t_timer.pool.push_back(pe.second);
         t_timer.cur.clear();
+    });
+}
+
+int galario_stage_repeat(int stage, int reps) {
+    return guard([&] {
+        if (reps < 1) invalid("stage_repeat", "reps must be >= 1");
+        if (stage != ST_ROW && stage != ST_COL && stage != ST_SAMPLE && stage != -1)
+            invalid("stage_repeat", "only the row pass (0), column pass (1) and forward sample (2) repeat");
+        t_timer.rep_stage = stage;
+        t_timer.rep_n = reps;
     });
 }
 

@@ -90,12 +90,19 @@ def take_oob_flag() -> bool:
     return bool(out.value)
 
 
-STAGES = ('row_pass', 'column_pass', 'sample', 'adjoint_column_pass', 'adjoint_row_pass')
+STAGES = ('row_pass', 'column_pass', 'sample', 'adjoint_column_pass', 'adjoint_row_pass', 'reduce')
 
 
 def stage_timer(enable: bool) -> None:
     """Start (True) / stop (False) per-stage HIP-event timing of this thread's device calls."""
     _check(lib.galario_stage_timer(int(bool(enable))))
+
+
+def stage_repeat(stage, reps: int) -> None:
+    """While timing, launch one forward kernel ('row_pass', 'column_pass' or 'sample'; None for none)
+    `reps` times back to back, so its stage time / reps is its average launch duration."""
+    idx = -1 if stage is None else STAGES.index(stage)
+    _check(lib.galario_stage_repeat(idx, int(reps)))
 
 
 def stage_times():

@@ -207,10 +207,17 @@ int galario_chi2_profile_grad_dev_f32(void *stream, int nr, const float *intensi
 /* Per-stage timing of the calling thread's _dev calls (no counterpart in
  * galario, whose GALARIO_TIMING build prints cudaEvent timings per stage,
  * src/galario.cpp:275-315): enable=1 starts recording a HIP event before each
- * stage (0 row pass, 1 column pass, 2 sample + chi2 reduction, 3 adjoint column
- * pass, 4 adjoint row pass + profile adjoint) and after the last,
- * on the call's stream; enable=0 stops.  Either clears what was recorded. */
+ * stage (0 row pass, 1 column pass, 2 sample (forward, or forward and
+ * corner-gradient scatter), 3 adjoint column pass, 4 adjoint row pass +
+ * profile adjoint, 5 final reduction of the per-workgroup partials) and after
+ * the last, on the call's stream; enable=0 stops.  Either clears what was
+ * recorded. */
 int galario_stage_timer(int enable);
+/* While the stage timer is on, launch the kernel of `stage` (0 row pass,
+ * 1 column pass, 2 forward sample; -1 none) `reps` times back to back inside
+ * its timed interval, so that the interval / reps is that kernel's average
+ * launch duration.  The three kernels are idempotent; results are unchanged. */
+int galario_stage_repeat(int stage, int reps);
 /* Waits for the recorded events and writes the summed milliseconds of each
  * stage to ms[0..nstage-1] and the number of timed calls to *ncalls. */
 int galario_stage_times(double *ms, int nstage, int *ncalls);

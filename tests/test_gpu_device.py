@@ -93,3 +93,28 @@ def test_stage_timer_records_calls():
     ta.stage_timer(False)
     assert n == 3 and all(ms[k] > 0 for k in ('row_pass', 'column_pass', 'sample'))
     assert ms['adjoint_column_pass'] == 0 and ms['adjoint_row_pass'] == 0
+
+
+def test_stage_repeat_leaves_results_unchanged():
+    """Repeated launches of the forward kernels (kernel timing) are idempotent."""
+    from galario_amd import torch_api as ta
+    d = workload(nvis=10_000)
+    dev = torch.device('cuda:0')
+    ds = ta.Dataset(d['u'], d['v'], d['re'], d['im'], d['w'], device=dev)
+    tI = torch.from_numpy(d['I']).to(dev)
+    args = (tI, d['Rmin'], d['dR'], d['nxy'], d['dxy'])
+    ref = ds.chi2_profile(*args, check_central_pixel=False, **d['kw']).item()
+    vref = ds.sample_profile(*args, check_central_pixel=False, **d['kw']).cpu().numpy()
+    ta.stage_timer(True)
+    try:
+        for st in ('row_pass', 'column_pass', 'sample'):
+            ta.stage_repeat(st, 3)
+            assert ds.chi2_profile(*args, check_central_pixel=False, **d['kw']).item() == ref
+            assert np.array_equal(ds.sample_profile(*args, check_central_pixel=False, **d['kw']).cpu().numpy(), vref)
+        ms, n = ta.stage_times()
+        assert n == 6 and ms['reduce'] > 0
+    finally:
+        ta.stage_repeat(None, 1)
+        ta.stage_timer(False)
+    with pytest.raises(ValueError):
+        ta.stage_repeat('adjoint_row_pass', 2)
