@@ -38,6 +38,21 @@ def _preload_torch_hip_runtime() -> None:
             return
 
 
+class DatasetF64(ctypes.Structure):
+    """galario_dataset_f64 (include/galario_amd.h)."""
+    _fields_ = [('nxy', ctypes.c_int), ('dxy', ctypes.c_double), ('duv', ctypes.c_double), ('nd', ctypes.c_int),
+                ('u', _vp), ('v', _vp), ('vis_obs_re', _vp), ('vis_obs_im', _vp), ('weights', _vp)]
+
+
+class DatasetF32(ctypes.Structure):
+    """galario_dataset_f32 (include/galario_amd.h)."""
+    _fields_ = [('nxy', ctypes.c_int), ('dxy', ctypes.c_float), ('duv', ctypes.c_float), ('nd', ctypes.c_int),
+                ('u', _vp), ('v', _vp), ('vis_obs_re', _vp), ('vis_obs_im', _vp), ('weights', _vp)]
+
+
+BATCH_EAGER = 1
+
+
 def _load():
     _preload_torch_hip_runtime()
     if not os.path.exists(LIB_PATH):
@@ -58,8 +73,21 @@ def _load():
     lib.galario_stage_timer.argtypes = [_c_int]
     lib.galario_stage_repeat.argtypes = [_c_int, _c_int]
     lib.galario_stage_times.argtypes = [ctypes.POINTER(ctypes.c_double), _c_int, _ip]
-    for suf, real in (('f64', ctypes.c_double), ('f32', ctypes.c_float)):
+    lib.galario_batch_destroy.argtypes = [_vp]
+    lib.galario_batch_info.argtypes = [_vp, _c_int, _c_int, _ip, _ip]
+    lib.galario_batch_set_chunk.argtypes = [_vp, _c_int, _c_int, _c_int]
+    lib.galario_batch_take_oob_flag.argtypes = [_vp, _ip]
+    for suf, real, ds in (('f64', ctypes.c_double, DatasetF64), ('f32', ctypes.c_float, DatasetF32)):
         R = real
+        f = getattr(lib, f'galario_batch_create_{suf}')
+        f.argtypes = [_c_int, ctypes.POINTER(ds), _c_int, R, R, _c_int, _c_int, ctypes.POINTER(_vp)]
+        f.restype = _c_int
+        f = getattr(lib, f'galario_batch_chi2_profile_{suf}')
+        f.argtypes = [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _c_int]
+        f.restype = _c_int
+        f = getattr(lib, f'galario_batch_chi2_profile_grad_{suf}')
+        f.argtypes = [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int]
+        f.restype = _c_int
         sig = {
             'sample_profile': [_c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp, _c_int],
             'sample_image': [_c_int, _c_int, _vp, R, R, R, R, R, _c_int, _vp, _vp, _vp],

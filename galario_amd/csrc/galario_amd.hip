@@ -12,7 +12,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -23,6 +25,7 @@
 
 #include "galario_amd.h"
 #include "gl_adjoint.hpp"
+#include "gl_batch.hpp"
 #include "gl_kernels.hpp"
 
 using namespace gl;
@@ -974,7 +977,7 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         sa.flag = w.flag_dev.get<unsigned>(1);
         const unsigned nb = blocks_for(size_t(nd), 256);
         double* part = w.part.get<double>(size_t(4) * (nb ? nb : 1));
-        GradArgs<T> ga{Hh, hot, part};
+        GradArgs<T> ga{Hh, hot, part, int(nb)};
         t_timer.mark(ST_SAMPLE, s);
         w.Hh_zero = false;
         if (nb) hipLaunchKernelGGL((k_sample_grad_sym<T>), dim3(nb), dim3(256), 0, s, sa, ga);
@@ -996,6 +999,411 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         launch_adjrow_sym<T>(L, g.pairs_r, s, ara);
         HC(hipGetLastError());
         t_timer.end_call(s);
+    });
+}
+
+// ====================================================================== batched form
+// A plan holds the datasets (device pointers owned by the caller), their
+// grouping by grid (nxy, dxy): one spectrum per walker and group, sampled by
+// every dataset of the group), slot buffers for a chunk of walkers, and per
+// (number of walkers, gradient) the chunk size picked by timing and the
+// captured graph.  Calls copy the caller's parameters into plan-owned
+// buffers, launch the graph and copy the results out, all on the caller's
+// stream.
+constexpr uint32_t BATCH_MAGIC = 0x67616c62u;
+constexpr size_t BATCH_SLOT_BUDGET = size_t(16) << 30;   // bytes of slot buffers at most
+
+struct BatchPlanBase {
+    uint32_t magic = BATCH_MAGIC;
+    int prec = 0;          // 8 double, 4 float
+    virtual ~BatchPlanBase() = default;
+};
+
+template <typename T>
+struct BatchSet {
+    int nxy;
+    T dxy, duv;
+    int nd;
+    const T *u, *v, *re, *im, *w;
+};
+
+template <typename T>
+struct BatchGroup {
+    int N;
+    T dxy;
+    SymGeom g;
+    SweepPrm<T> sw;        // shared scalars (I and cos_inc are per walker)
+    std::vector<int> sets;
+};
+
+template <typename T>
+struct BatchPlan : BatchPlanBase {
+    int device = 0;
+    int nr = 0, max_w = 0;
+    std::vector<BatchSet<T>> sets;
+    std::vector<int> nb, off;
+    int ldPart = 1;
+    std::vector<BatchGroup<T>> groups;
+    size_t sTs = 0, sH = 0, sHh = 0, hot_off = 0, sTh = 0;
+    int slots_ts = 0, slots_grad = 0;   // slots allocated (forward buffers, gradient buffers)
+    bool Hh_zero = false;
+    DevBuf wp, I, geom, part, Ts, H, Hh, Th, out_sets, out_tot, gI, ggeom, flag;
+    hipStream_t cap = nullptr;
+    hipStream_t last = nullptr;   // stream of the previous call (its buffers are reused)
+    bool has_last = false;
+    hipEvent_t ev = nullptr;
+    struct Exec {
+        int chunk = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<std::pair<int, int>, Exec> runs;   // (nwalkers, grad)
+    std::mutex mu;
+
+    ~BatchPlan() override {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(device);
+        (void)hipDeviceSynchronize();   // work of the last call may still read the buffers
+        for (auto& kv : runs)
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        for (DevBuf* b : {&wp, &I, &geom, &part, &Ts, &H, &Hh, &Th, &out_sets, &out_tot, &gI, &ggeom, &flag})
+            b->release();
+        if (cap) (void)hipStreamDestroy(cap);
+        if (ev) (void)hipEventDestroy(ev);
+        (void)hipSetDevice(cur);
+        magic = 0;
+    }
+
+    size_t slot_bytes(bool grad) const {
+        return sizeof(T) * (sTs + sH) + (grad ? sizeof(double) * sHh + sizeof(T) * sTh : 0);
+    }
+    int max_chunk(bool grad, int W) const {
+        const size_t c = BATCH_SLOT_BUDGET / slot_bytes(grad);
+        return int(std::max<size_t>(1, std::min<size_t>(c, size_t(W))));
+    }
+    // drop every captured graph (they hold the addresses of the slot buffers)
+    void drop_graphs() {
+        for (auto& kv : runs)
+            if (kv.second.exec) {
+                (void)hipGraphExecDestroy(kv.second.exec);
+                kv.second.exec = nullptr;
+            }
+    }
+    // grow slot buffers to `chunk` slots (outside any capture)
+    void ensure_slots(int chunk, bool grad, hipStream_t s) {
+        if (chunk > slots_ts || (grad && chunk > slots_grad)) {
+            HC(hipStreamSynchronize(s));   // in-flight work may still use the buffers being replaced
+            drop_graphs();
+        }
+        if (chunk > slots_ts) {
+            Ts.get<T>(sTs * chunk);
+            H.get<T>(sH * chunk);
+            slots_ts = chunk;
+        }
+        if (grad && chunk > slots_grad) {
+            Hh.get<double>(sHh * chunk);
+            Th.get<T>(sTh * chunk);
+            slots_grad = chunk;
+            Hh_zero = false;
+        }
+        if (grad && !Hh_zero) {
+            const size_t n = sHh * size_t(slots_grad);
+            hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(n, 256)), dim3(256), 0, s, Hh.get<double>(n), n);
+            HC(hipGetLastError());
+            Hh_zero = true;
+        }
+    }
+};
+
+template <typename T>
+BatchPtrs<T> batch_ptrs(BatchPlan<T>& P, bool grad) {
+    BatchPtrs<T> b{};
+    b.I = static_cast<const T*>(P.I.p);
+    b.nr = P.nr;
+    b.wp = static_cast<const WalkerPrm<T>*>(P.wp.p);
+    b.Ts = static_cast<T*>(P.Ts.p);
+    b.sTs = P.sTs;
+    b.H = static_cast<T*>(P.H.p);
+    b.sH = P.sH;
+    if (grad) {
+        b.Hh = static_cast<double*>(P.Hh.p);
+        b.sHh = P.sHh;
+        b.hot_off = P.hot_off;
+        b.Th = static_cast<T*>(P.Th.p);
+        b.sTh = P.sTh;
+        b.gI = static_cast<double*>(P.gI.p);
+        b.ggeom = static_cast<double*>(P.ggeom.p);
+    }
+    b.part = static_cast<double*>(P.part.p);
+    b.ldPart = P.ldPart;
+    return b;
+}
+
+// Enqueue one batched evaluation of W walkers (parameters already in P.I /
+// P.geom) in chunks of `chunk` walkers; only kernel launches (capturable).
+template <typename T>
+void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad) {
+    const int nr = P.nr;
+    const size_t nz = std::max(size_t(W), grad ? size_t(W) * size_t(std::max(nr, 4)) : size_t(W));
+    hipLaunchKernelGGL((k_batch_setup<T>), dim3(blocks_for(nz, 256)), dim3(256), 0, s, W,
+                       static_cast<const T*>(P.geom.p), static_cast<WalkerPrm<T>*>(P.wp.p),
+                       grad ? static_cast<double*>(P.gI.p) : nullptr, nr,
+                       grad ? static_cast<double*>(P.ggeom.p) : nullptr);
+    BatchPtrs<T> b = batch_ptrs(P, grad);
+    unsigned* flag = static_cast<unsigned*>(P.flag.p);
+    for (int w0 = 0; w0 < W; w0 += chunk) {
+        const int nw = std::min(chunk, W - w0);
+        b.w0 = w0;
+        for (const BatchGroup<T>& G : P.groups) {
+            const SymGeom& g = G.g;
+            const int L = g.L;
+            const cpx<T>* tw = twiddles<T>(G.N);
+            SymRowArgs<T> ra{G.sw, G.N, g.Rs, g.nrows, g.ldT, tw, nullptr, nullptr, 0, nullptr};
+            SymColArgs<T> ca{nullptr, G.N, g.Rs, g.ldT, tw, nullptr, g.ldH};
+            const int cblocks = 8 * ((g.pairs_c + 7) / 8);
+#define GL_CALL(LL)                                                                                              \
+    hipLaunchKernelGGL((k_rowpass_sym_b<T, LL>), dim3(g.pairs_r, nw), dim3(FftShape<LL>::NT), 0, s, ra, b);      \
+    hipLaunchKernelGGL((k_colpass_sym_b<T, LL>), dim3(cblocks, nw), dim3(FftShape<LL>::NT), 0, s, ca, b)
+            GL_L_SWITCH(T, GL_CALL)
+#undef GL_CALL
+            for (int si : G.sets) {
+                const BatchSet<T>& d = P.sets[si];
+                if (P.nb[si] == 0) continue;
+                SampleArgs<T> a{};
+                a.ldH = g.ldH;
+                a.nrow = G.N;
+                a.ncol = g.C;
+                a.v_origin = T(1);
+                a.duv = d.duv;
+                a.nd = d.nd;
+                a.u = d.u;
+                a.v = d.v;
+                a.re = d.re;
+                a.im = d.im;
+                a.w = d.w;
+                a.flag = flag;
+                b.partOff = P.off[si];
+                if (grad)
+                    hipLaunchKernelGGL((k_sample_grad_sym_b<T>), dim3(P.nb[si], nw), dim3(256), 0, s, a, b);
+                else
+                    hipLaunchKernelGGL((k_sample_b<T>), dim3(P.nb[si], nw), dim3(256), 0, s, a, b);
+            }
+            if (grad) {
+                AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0};
+                aca.ldTh = int(P.sTh / size_t(g.nrows));
+                AdjRowArgs<T> ara{G.sw, nullptr, aca.ldTh, G.N, g.Rs, g.nrows, tw, 0.0, nullptr, nullptr};
+#define GL_CALL(LL)                                                                                              \
+    hipLaunchKernelGGL((k_adjcol_sym_b<T, LL>), dim3(cblocks, nw), dim3(FftShape<LL>::NT), 0, s, aca, b);        \
+    hipLaunchKernelGGL((k_adjrow_sym_b<T, LL>), dim3(g.pairs_r, nw), dim3(FftShape<LL>::NT), 0, s, ara, b)
+                GL_L_SWITCH(T, GL_CALL)
+#undef GL_CALL
+            }
+        }
+    }
+    BatchSets bs{};
+    bs.n = int(P.sets.size());
+    for (int i = 0; i < bs.n; ++i) {
+        bs.off[i] = P.off[size_t(i)];
+        bs.nb[i] = P.nb[size_t(i)];
+    }
+    hipLaunchKernelGGL(k_batch_reduce, dim3(W), dim3(1024), 0, s, static_cast<const double*>(P.part.p), P.ldPart, bs,
+                       grad ? 4 : 1, static_cast<double*>(P.out_sets.p), static_cast<double*>(P.out_tot.p),
+                       grad ? static_cast<double*>(P.ggeom.p) : nullptr);
+    HC(hipGetLastError());
+}
+
+template <typename T>
+int batch_create(int nsets, const BatchSet<T>* sets, int nr, T Rmin, T dR, int max_w, int check_central,
+                 void** out) {
+    return guard([&] {
+        if (!out) invalid("batch", "null plan pointer");
+        *out = nullptr;
+        if (nsets < 1 || nsets > BATCH_MAX_SETS)
+            invalid("batch", "number of datasets = " + std::to_string(nsets) + " (expect 1 to " +
+                                 std::to_string(BATCH_MAX_SETS) + ")");
+        if (!sets) invalid("batch", "null dataset array");
+        if (max_w < 1) invalid("batch", "max_walkers must be >= 1");
+        auto P = std::make_unique<BatchPlan<T>>();
+        P->prec = int(sizeof(T));
+        P->nr = nr;
+        P->max_w = max_w;
+        size_t blocks = 0;
+        for (int i = 0; i < nsets; ++i) {
+            const BatchSet<T>& d = sets[i];
+            check_input(d.nxy);
+            check_nd(d.nd);
+            if (d.nd > 0 && (!d.u || !d.v || !d.re || !d.im || !d.w))
+                invalid("batch", "dataset " + std::to_string(i) + " has a null array");
+            if (!(d.duv > T(0))) invalid("batch", "dataset " + std::to_string(i) + ": expect duv > 0");
+            const int L = fast_log2<T>(d.nxy);
+            if (L < 0)
+                invalid("batch", "dataset " + std::to_string(i) + ": nxy = " + std::to_string(d.nxy) +
+                                     " is not supported by the batched form (power of two from 128 to " +
+                                     std::to_string(1 << max_fast_log2<T>()) + ")");
+            // same grid (nxy, dxy): share the group's spectrum
+            int gi = -1;
+            for (size_t k = 0; k < P->groups.size(); ++k)
+                if (P->groups[k].N == d.nxy && P->groups[k].dxy == d.dxy) gi = int(k);
+            if (gi < 0) {
+                BatchGroup<T> G{};
+                ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, d.dxy, d.nxy, T(0), check_central);
+                G.N = d.nxy;
+                G.dxy = d.dxy;
+                G.sw = ps.sw;
+                G.sw.I = nullptr;
+                G.g = sym_geom(L, d.nxy, ps.rmax);
+                P->groups.push_back(G);
+                gi = int(P->groups.size()) - 1;
+            }
+            P->groups[size_t(gi)].sets.push_back(i);
+            P->sets.push_back(d);
+            const int nb = int(blocks_for(size_t(d.nd), 256));
+            P->off.push_back(int(blocks));
+            P->nb.push_back(nb);
+            blocks += size_t(nb);
+        }
+        P->ldPart = int(std::max<size_t>(blocks, 1));
+        HC(hipGetDevice(&P->device));   // (all argument checks above run without a GPU)
+        size_t maxHh = 0, maxTh = 0;
+        for (const BatchGroup<T>& G : P->groups) {
+            const SymGeom& g = G.g;
+            P->sTs = std::max(P->sTs, size_t(g.C + 1) * g.ldT);
+            P->sH = std::max(P->sH, size_t(g.C + 1) * g.ldH);
+            maxHh = std::max(maxHh, size_t(g.C + 1) * g.ldH);
+            const size_t ldTh = size_t((g.C + 1 + 7) & ~7);
+            maxTh = std::max(maxTh, ldTh);
+        }
+        P->hot_off = (maxHh + 31) & ~size_t(31);
+        P->sHh = P->hot_off + size_t(GHOT_ELEMS);
+        // adjoint rows: nrows x ldTh per group, one stride for all
+        size_t sTh = 0;
+        for (const BatchGroup<T>& G : P->groups) sTh = std::max(sTh, size_t(G.g.nrows) * maxTh);
+        P->sTh = sTh;
+        // parameters, walker scalars, partials, outputs
+        P->I.template get<T>(size_t(max_w) * nr);
+        P->geom.template get<T>(size_t(max_w) * 4);
+        P->wp.template get<WalkerPrm<T>>(size_t(max_w));
+        P->part.template get<double>(size_t(max_w) * 4 * P->ldPart);
+        P->out_sets.template get<double>(size_t(max_w) * nsets);
+        P->out_tot.template get<double>(size_t(max_w));
+        P->gI.template get<double>(size_t(max_w) * nr);
+        P->ggeom.template get<double>(size_t(max_w) * 4);
+        unsigned* f = P->flag.template get<unsigned>(1);
+        HC(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
+        HC(hipEventCreateWithFlags(&P->ev, hipEventDisableTiming));
+        HC(hipMemsetAsync(f, 0, sizeof(unsigned), P->cap));
+        HC(hipStreamSynchronize(P->cap));
+        *out = static_cast<BatchPlanBase*>(P.release());
+    });
+}
+
+template <typename T>
+BatchPlan<T>& plan_of(void* h) {
+    auto* b = static_cast<BatchPlanBase*>(h);
+    if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
+    if (b->prec != int(sizeof(T))) invalid("batch", "plan precision does not match the entry point");
+    auto& P = *static_cast<BatchPlan<T>*>(b);
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    if (dev != P.device) invalid("batch", "the plan belongs to device " + std::to_string(P.device));
+    return P;
+}
+
+// Average time of one eager evaluation at `chunk` (after one warm-up).
+template <typename T>
+float batch_time(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad) {
+    P.ensure_slots(chunk, grad, s);
+    batch_enqueue(P, s, W, chunk, grad);
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    HC(hipEventRecord(e0, s));
+    const int reps = 2;
+    for (int r = 0; r < reps; ++r) batch_enqueue(P, s, W, chunk, grad);
+    HC(hipEventRecord(e1, s));
+    HC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+}
+
+// Chunk size for W walkers: the fastest power of two (and W itself) within
+// the slot budget, by timing eager evaluations on the caller's stream.
+template <typename T>
+int batch_tune(BatchPlan<T>& P, hipStream_t s, int W, bool grad) {
+    const int cmax = P.max_chunk(grad, W);
+    std::vector<int> cand;
+    for (int c = 1; c < cmax; c *= 2) cand.push_back(c);
+    cand.push_back(cmax);
+    int best = cand.back();
+    float tbest = 0.f;
+    for (int c : cand) {
+        const float t = batch_time(P, s, W, c, grad);
+        if (c == cand.front() || t < tbest) {
+            tbest = t;
+            best = c;
+        }
+    }
+    return best;
+}
+
+template <typename T>
+int batch_run(void* h, void* stream, int W, const T* I, const T* geom, double* chi2_sets, double* chi2,
+              double* gI, double* ggeom, int flags, bool grad) {
+    return guard([&] {
+        BatchPlan<T>& P = plan_of<T>(h);
+        std::lock_guard<std::mutex> lk(P.mu);
+        if (W < 1 || W > P.max_w)
+            invalid("batch", "nwalkers = " + std::to_string(W) + " (plan holds 1 to " + std::to_string(P.max_w) + ")");
+        if (!I || !geom) invalid("batch", "null parameter array");
+        if (grad && (!gI || !ggeom)) invalid("batch", "null gradient output");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (P.has_last && P.last != s) {
+            HC(hipEventRecord(P.ev, P.last));
+            HC(hipStreamWaitEvent(s, P.ev, 0));
+        }
+        P.last = s;
+        P.has_last = true;
+        if (I != P.I.p)
+            HC(hipMemcpyAsync(P.I.p, I, sizeof(T) * size_t(W) * P.nr, hipMemcpyDeviceToDevice, s));
+        if (geom != P.geom.p)
+            HC(hipMemcpyAsync(P.geom.p, geom, sizeof(T) * size_t(W) * 4, hipMemcpyDeviceToDevice, s));
+        const auto key = std::make_pair(W, grad ? 1 : 0);
+        auto& run = P.runs[key];
+        if (run.chunk == 0) run.chunk = batch_tune(P, s, W, grad);
+        P.ensure_slots(run.chunk, grad, s);
+        if (flags & GALARIO_BATCH_EAGER) {
+            batch_enqueue(P, s, W, run.chunk, grad);
+        } else {
+            if (!run.exec) {
+                // capture on the plan's stream (nothing runs during capture)
+                HC(hipStreamSynchronize(s));
+                hipGraph_t graph = nullptr;
+                HC(hipStreamBeginCapture(P.cap, hipStreamCaptureModeThreadLocal));
+                try {
+                    batch_enqueue(P, P.cap, W, run.chunk, grad);
+                } catch (...) {
+                    (void)hipStreamEndCapture(P.cap, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    throw;
+                }
+                HC(hipStreamEndCapture(P.cap, &graph));
+                const hipError_t e = hipGraphInstantiate(&run.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HC(e);
+            }
+            HC(hipGraphLaunch(run.exec, s));
+        }
+        const size_t S = P.sets.size();
+        if (chi2_sets)
+            HC(hipMemcpyAsync(chi2_sets, P.out_sets.p, sizeof(double) * size_t(W) * S, hipMemcpyDeviceToDevice, s));
+        if (chi2) HC(hipMemcpyAsync(chi2, P.out_tot.p, sizeof(double) * size_t(W), hipMemcpyDeviceToDevice, s));
+        if (grad) {
+            HC(hipMemcpyAsync(gI, P.gI.p, sizeof(double) * size_t(W) * P.nr, hipMemcpyDeviceToDevice, s));
+            HC(hipMemcpyAsync(ggeom, P.ggeom.p, sizeof(double) * size_t(W) * 4, hipMemcpyDeviceToDevice, s));
+        }
     });
 }
 
@@ -1124,6 +1532,95 @@ GL_BOTH(GL_MAIN)
     }
 
 GL_BOTH(GL_GRAD)
+
+#define GL_BATCH(SUF, T)                                                                                          \
+    int galario_batch_create_##SUF(int nsets, const galario_dataset_##SUF* sets, int nr, T Rmin, T dR,           \
+                                   int max_walkers, int check_central, void** plan) {                            \
+        std::vector<BatchSet<T>> v;                                                                               \
+        if (sets && nsets > 0 && nsets <= BATCH_MAX_SETS)                                                         \
+            for (int i = 0; i < nsets; ++i)                                                                       \
+                v.push_back(BatchSet<T>{sets[i].nxy, sets[i].dxy, sets[i].duv, sets[i].nd, sets[i].u, sets[i].v, \
+                                        sets[i].vis_obs_re, sets[i].vis_obs_im, sets[i].weights});               \
+        return batch_create<T>(nsets, v.empty() ? nullptr : v.data(), nr, Rmin, dR, max_walkers, check_central,  \
+                               plan);                                                                             \
+    }                                                                                                             \
+    int galario_batch_chi2_profile_##SUF(void* plan, void* stream, int nwalkers, const T* I, const T* geom,      \
+                                         double* chi2_sets, double* chi2, int flags) {                            \
+        return batch_run<T>(plan, stream, nwalkers, I, geom, chi2_sets, chi2, nullptr, nullptr, flags, false);    \
+    }                                                                                                             \
+    int galario_batch_chi2_profile_grad_##SUF(void* plan, void* stream, int nwalkers, const T* I, const T* geom, \
+                                              double* chi2_sets, double* chi2, double* grad_I, double* grad_geom, \
+                                              int flags) {                                                        \
+        return batch_run<T>(plan, stream, nwalkers, I, geom, chi2_sets, chi2, grad_I, grad_geom, flags, true);    \
+    }
+
+GL_BOTH(GL_BATCH)
+
+int galario_batch_destroy(void* plan) {
+    return guard([&] {
+        if (!plan) return;
+        auto* b = static_cast<BatchPlanBase*>(plan);
+        if (b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
+        delete b;
+    });
+}
+
+int galario_batch_info(void* plan, int nwalkers, int grad, int* chunk, int* graph_ready) {
+    return guard([&] {
+        auto* b = static_cast<BatchPlanBase*>(plan);
+        if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
+        int c = 0, g = 0;
+        auto look = [&](auto& P) {
+            std::lock_guard<std::mutex> lk(P.mu);
+            auto it = P.runs.find(std::make_pair(nwalkers, grad ? 1 : 0));
+            if (it != P.runs.end()) {
+                c = it->second.chunk;
+                g = it->second.exec != nullptr;
+            }
+        };
+        if (b->prec == 8) look(*static_cast<BatchPlan<double>*>(b));
+        else look(*static_cast<BatchPlan<float>*>(b));
+        if (chunk) *chunk = c;
+        if (graph_ready) *graph_ready = g;
+    });
+}
+
+int galario_batch_set_chunk(void* plan, int nwalkers, int grad, int chunk) {
+    return guard([&] {
+        auto* b = static_cast<BatchPlanBase*>(plan);
+        if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
+        auto set = [&](auto& P) {
+            std::lock_guard<std::mutex> lk(P.mu);
+            if (nwalkers < 1 || nwalkers > P.max_w) invalid("batch", "nwalkers out of range");
+            if (chunk < 1 || chunk > P.max_chunk(grad != 0, nwalkers))
+                invalid("batch", "chunk = " + std::to_string(chunk) + " (expect 1 to " +
+                                     std::to_string(P.max_chunk(grad != 0, nwalkers)) + ")");
+            auto& run = P.runs[std::make_pair(nwalkers, grad ? 1 : 0)];
+            if (run.exec) (void)hipGraphExecDestroy(run.exec);
+            run.exec = nullptr;
+            run.chunk = chunk;
+        };
+        if (b->prec == 8) set(*static_cast<BatchPlan<double>*>(b));
+        else set(*static_cast<BatchPlan<float>*>(b));
+    });
+}
+
+int galario_batch_take_oob_flag(void* plan, int* out) {
+    return guard([&] {
+        auto* b = static_cast<BatchPlanBase*>(plan);
+        if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
+        auto take = [&](auto& P) {
+            std::lock_guard<std::mutex> lk(P.mu);
+            unsigned h = 0;
+            if (P.has_last) HC(hipStreamSynchronize(P.last));
+            HC(hipMemcpy(&h, P.flag.p, sizeof(unsigned), hipMemcpyDeviceToHost));
+            if (h) HC(hipMemset(P.flag.p, 0, sizeof(unsigned)));
+            if (out) *out = h ? 1 : 0;
+        };
+        if (b->prec == 8) take(*static_cast<BatchPlan<double>*>(b));
+        else take(*static_cast<BatchPlan<float>*>(b));
+    });
+}
 
 int galario_stage_timer(int enable) {
     return guard([&] {

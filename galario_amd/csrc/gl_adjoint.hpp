@@ -58,7 +58,8 @@ template <typename T>
 struct GradArgs {
     double* Hh;          // adjoint of H, same layout (column stride ldH), zero on entry
     double* hot;         // GREP x GHOT x GHOT replicas [rep][col][row], zero on entry
-    double* partials;    // 4 x nblocks: chi2, dPA, ddRA, ddDec
+    double* partials;    // 4 streams of per-workgroup partials: chi2, dPA, ddRA, ddDec
+    int ldp;             // stride between the streams (>= number of workgroups)
 };
 
 __device__ __forceinline__ void lds_add_f64(double* p, double v) {
@@ -78,13 +79,13 @@ __device__ __forceinline__ int wave_max(int x) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_sample_grad_sym(SampleArgs<T> a, GradArgs<T> g) {
+__device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, const GradArgs<T>& g, int blk) {
     __shared__ double win[GWIN];
     __shared__ double hot[GHOT * GHOT];
     __shared__ double red[4][4];
     __shared__ int box[4][4];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int i = blockIdx.x * 256 + tid;
+    const int i = blk * 256 + tid;
     double c2 = 0.0, gpa = 0.0, gra = 0.0, gde = 0.0;
     double gk[4] = {0.0, 0.0, 0.0, 0.0};         // corners (fl_u, m0), (fl_u, m1), (fu1, m0), (fu1, m1)
     int cu0 = 0, cu1 = 0, cm0 = 0, cm1 = 0;
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(256) void k_sample_grad_sym(SampleArgs<T> a, GradAr
     }
     __syncthreads();
     {
-        double* rep = g.hot + size_t(blockIdx.x % GREP) * (GHOT * GHOT);
+        double* rep = g.hot + size_t(blk % GREP) * (GHOT * GHOT);
         for (int j = tid; j < GHOT * GHOT; j += 256) {
             const double x = hot[j];
             if (x != 0.0) atomic_add_f64(rep + j, x);
@@ -205,7 +206,12 @@ __global__ __launch_bounds__(256) void k_sample_grad_sym(SampleArgs<T> a, GradAr
         red[3][wv] = gde;
     }
     __syncthreads();
-    if (tid < 4) g.partials[size_t(tid) * gridDim.x + blockIdx.x] = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
+    if (tid < 4) g.partials[size_t(tid) * g.ldp + blk] = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sample_grad_sym(SampleArgs<T> a, GradArgs<T> g) {
+    sample_grad_sym_body<T>(a, g, blockIdx.x);
 }
 
 // Deterministic sums of 4 streams of n partials: *dst[j] = sum_i p[j*n + i].
@@ -249,14 +255,14 @@ __device__ __forceinline__ double mult(int m, int N) { return (m == 0 || m == N 
 __device__ __forceinline__ double ext_weight(int k, int N) { return (k == 0 || k == N / 2) ? 1.0 : 0.5; }
 
 template <typename T, int LOG2N>
-__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym(AdjColArgs<T> a) {
+__device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk, int nblk) {
     using S = FftShape<LOG2N>;
     using F = BlockFft<T, LOG2N, false>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
     constexpr int NPAIR = (C + 1) / 2;
     __shared__ T lds[S::LDS_ELEMS];
-    const int pair = xcd_column(blockIdx.x, gridDim.x);
+    const int pair = xcd_column(blk, nblk);
     if (pair >= NPAIR) return;
     const int la = 2 * pair, lb = la + 1;
     const bool has_b = lb < C;
@@ -301,6 +307,11 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym(AdjColArgs<T
             if (has_b) o[1] = cm * v[r].im;
         }
     }
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym(AdjColArgs<T> a) {
+    adjcol_sym_body<T, LOG2N>(a, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -365,14 +376,14 @@ __device__ __noinline__ void central_adjoint(double gz, const SweepPrm<T>& p, do
 }
 
 template <typename T, int LOG2N>
-__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjrow_sym(AdjRowArgs<T> a) {
+__device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk) {
     using S = FftShape<LOG2N>;
     using F = BlockFft<T, LOG2N, false>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
     __shared__ T lds[S::LDS_ELEMS];
     __shared__ double red[16];
     const int tid = threadIdx.x;
-    const int ma = 2 * blockIdx.x, mb = ma + 1;
+    const int ma = 2 * blk, mb = ma + 1;
     const bool hb = mb < a.nrows;
     const T* ra = a.Th + size_t(ma) * a.ldTh;
     const T* rb = ra + a.ldTh;
@@ -396,6 +407,11 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjrow_sym(AdjRowArgs<T
     }
     const double b = block_sum_any(ginc, red);
     if (tid == 0 && b != 0.0) atomic_add_f64(a.ginc, b);
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjrow_sym(AdjRowArgs<T> a) {
+    adjrow_sym_body<T, LOG2N>(a, blockIdx.x);
 }
 
 }  // namespace gl

@@ -1,0 +1,203 @@
+// Batched chi2Profile: many walkers' parameter sets (profile, inc, PA, dRA,
+// dDec) over several datasets.  galario has no batched form (its callers loop
+// over chi2Profile calls, one per walker and dataset); these kernels run the
+// same per-call arithmetic with the walker taken from blockIdx.y:
+//
+//   * a setup kernel computes each walker's scalars from its (inc, PA, dRA,
+//     dDec) with the formulas the single-call path evaluates on the host
+//     (create_image_h's cos(inc), uv_rotate_h's cos/sin(PA) and rotated
+//     offsets, apply_phase_sampled_h's skip test and 2 pi scaling);
+//   * the row pass, column pass, sample and adjoint kernels are the bodies of
+//     the single-call kernels with per-walker pointers (a slot of the chunk
+//     of walkers being processed holds that walker's rows and spectrum);
+//   * per-workgroup partials land in [walker][stream][block] and one
+//     workgroup per walker sums them in the fixed order of the single-call
+//     reduction, dataset by dataset.
+#pragma once
+
+#include "gl_adjoint.hpp"
+
+namespace gl {
+
+template <typename T>
+struct WalkerPrm {
+    T cos_inc;
+    int rotate;
+    T cosPA, sinPA;
+    int phase;
+    T dRA2, dDec2;       // 2 pi dRArot, 2 pi dDecrot
+    double tan_inc;
+};
+
+// geom: [W][4] = inc, PA, dRA, dDec.  Also clears the gradient accumulators
+// gI[0..W*nr) and ggeom[0..4W) when given.
+template <typename T>
+__global__ __launch_bounds__(256) void k_batch_setup(int W, const T* __restrict__ geom, WalkerPrm<T>* wp, double* gI,
+                                                     int nr, double* ggeom) {
+    const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < size_t(W)) {
+        const T inc = geom[4 * i], PA = geom[4 * i + 1], dRA = geom[4 * i + 2], dDec = geom[4 * i + 3];
+        WalkerPrm<T> p;
+        p.cos_inc = T(cos(double(inc)));
+        p.tan_inc = tan(double(inc));
+        T dRArot, dDecrot;
+        if (PA == T(0)) {
+            p.rotate = 0;
+            p.cosPA = T(1);
+            p.sinPA = T(0);
+            dRArot = dRA;
+            dDecrot = dDec;
+        } else {
+            p.rotate = 1;
+            p.cosPA = T(cos(double(PA)));
+            p.sinPA = T(sin(double(PA)));
+            dRArot = dRA * p.cosPA - dDec * p.sinPA;
+            dDecrot = dRA * p.sinPA + dDec * p.cosPA;
+        }
+        p.phase = !(dRArot == T(0) && dDecrot == T(0));
+        p.dRA2 = T(double(dRArot) * (2. * double(T(M_PI))));
+        p.dDec2 = T(double(dDecrot) * (2. * double(T(M_PI))));
+        wp[i] = p;
+    }
+    if (gI && i < size_t(W) * size_t(nr)) gI[i] = 0.0;
+    if (ggeom && i < size_t(W) * 4) ggeom[i] = 0.0;
+}
+
+// Per-walker pointers: walker w = w0 + blockIdx.y uses slot blockIdx.y.
+template <typename T>
+struct BatchPtrs {
+    const T* I;              // profiles [W][nr]
+    int nr;
+    const WalkerPrm<T>* wp;  // [W]
+    int w0;
+    T* Ts;                   // rows of each slot (stride sTs)
+    size_t sTs;
+    T* H;                    // half spectrum of each slot (stride sH)
+    size_t sH;
+    double* Hh;              // adjoint half spectrum of each slot (stride sHh), zero between calls
+    size_t sHh;
+    size_t hot_off;          // hot-tile replicas at Hh + slot * sHh + hot_off
+    T* Th;                   // adjoint rows of each slot (stride sTh)
+    size_t sTh;
+    double* gI;              // [W][nr]
+    double* ggeom;           // [W][4]; the inc entry is accumulated here
+    double* part;            // [W][4][ldPart]
+    int ldPart;
+    int partOff;             // first block of the dataset being sampled
+};
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass_sym_b(SymRowArgs<T> a, BatchPtrs<T> b) {
+    const int slot = blockIdx.y, w = b.w0 + slot;
+    a.sw.I = b.I + size_t(w) * b.nr;
+    a.sw.cos_inc = b.wp[w].cos_inc;
+    a.Ts = b.Ts + slot * b.sTs;
+    a.zero = nullptr;
+    a.zero1 = nullptr;
+    rowpass_sym_body<T, LOG2N>(a, blockIdx.x);
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass_sym_b(SymColArgs<T> a, BatchPtrs<T> b) {
+    const int slot = blockIdx.y;
+    a.Ts = b.Ts + slot * b.sTs;
+    a.H = b.H + slot * b.sH;
+    colpass_sym_body<T, LOG2N>(a, blockIdx.x, gridDim.x);
+}
+
+template <typename T>
+__device__ __forceinline__ void walker_sample_args(SampleArgs<T>& a, const BatchPtrs<T>& b, int slot, int w) {
+    const WalkerPrm<T> p = b.wp[w];
+    a.H = b.H + slot * b.sH;
+    a.rotate = p.rotate;
+    a.cosPA = p.cosPA;
+    a.sinPA = p.sinPA;
+    a.phase = p.phase;
+    a.dRA2 = p.dRA2;
+    a.dDec2 = p.dDec2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sample_b(SampleArgs<T> a, BatchPtrs<T> b) {
+    const int slot = blockIdx.y, w = b.w0 + slot;
+    walker_sample_args(a, b, slot, w);
+    a.partials = b.part + size_t(w) * 4 * b.ldPart + b.partOff;
+    sample_body<T, MODE_CHI2, true>(a, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sample_grad_sym_b(SampleArgs<T> a, BatchPtrs<T> b) {
+    const int slot = blockIdx.y, w = b.w0 + slot;
+    walker_sample_args(a, b, slot, w);
+    double* hh = b.Hh + slot * b.sHh;
+    const GradArgs<T> g{hh, hh + b.hot_off, b.part + size_t(w) * 4 * b.ldPart + b.partOff, b.ldPart};
+    sample_grad_sym_body<T>(a, g, blockIdx.x);
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym_b(AdjColArgs<T> a, BatchPtrs<T> b) {
+    const int slot = blockIdx.y;
+    a.Hh = b.Hh + slot * b.sHh;
+    a.hot = a.Hh + b.hot_off;
+    a.Th = b.Th + slot * b.sTh;
+    adjcol_sym_body<T, LOG2N>(a, blockIdx.x, gridDim.x);
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjrow_sym_b(AdjRowArgs<T> a, BatchPtrs<T> b) {
+    const int slot = blockIdx.y, w = b.w0 + slot;
+    a.sw.I = b.I + size_t(w) * b.nr;
+    a.sw.cos_inc = b.wp[w].cos_inc;
+    a.Th = b.Th + slot * b.sTh;
+    a.tan_inc = b.wp[w].tan_inc;
+    a.gI = b.gI + size_t(w) * b.nr;
+    a.ginc = b.ggeom + size_t(w) * 4;
+    adjrow_sym_body<T, LOG2N>(a, blockIdx.x);
+}
+
+constexpr int BATCH_MAX_SETS = 32;
+
+struct BatchSets {
+    int n;
+    int off[BATCH_MAX_SETS];   // first block of each dataset in a partial stream
+    int nb[BATCH_MAX_SETS];    // blocks of each dataset
+};
+
+// One workgroup per walker: chi2 of each dataset (the single-call
+// k_sum_partials order), their sum over datasets, and with nstream = 4 the
+// PA, dRA, dDec gradients (k_sum_partials4 order per dataset, summed over
+// datasets).
+__global__ __launch_bounds__(1024) void k_batch_reduce(const double* __restrict__ part, int ldPart, BatchSets sets,
+                                                       int nstream, double* chi2_sets, double* chi2_tot,
+                                                       double* ggeom) {
+    __shared__ double red[16];
+    const int w = blockIdx.x;
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < sets.n; ++s) {
+        for (int k = 0; k < nstream; ++k) {
+            const double* p = part + (size_t(w) * 4 + k) * ldPart + sets.off[s];
+            double acc = 0.0;
+            for (int i = threadIdx.x; i < sets.nb[s]; i += 1024) acc += p[i];
+            acc = wave_sum(acc);
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double v = 0.0;
+                for (int q = 0; q < 16; ++q) v += red[q];
+                if (k == 0 && chi2_sets) chi2_sets[size_t(w) * sets.n + s] = v;
+                tot[k] += v;
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (chi2_tot) chi2_tot[w] = tot[0];
+        if (ggeom && nstream == 4) {
+            ggeom[size_t(w) * 4 + 1] = tot[1];
+            ggeom[size_t(w) * 4 + 2] = tot[2];
+            ggeom[size_t(w) * 4 + 3] = tot[3];
+        }
+    }
+}
+
+}  // namespace gl

@@ -306,16 +306,16 @@ struct SymRowArgs {
 };
 
 template <typename T, int LOG2N>
-__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass_sym(SymRowArgs<T> a) {
+__device__ __forceinline__ void rowpass_sym_body(const SymRowArgs<T>& a, int blk) {
     using S = FftShape<LOG2N>;
     using F = BlockFft<T, LOG2N, false>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
     __shared__ T lds[S::LDS_ELEMS];
     const int tid = threadIdx.x;
-    const int ma = 2 * blockIdx.x, mb = ma + 1;
+    const int ma = 2 * blk, mb = ma + 1;
     const bool hb = mb < a.nrows;
-    if (blockIdx.x == 0) {
+    if (blk == 0) {
         if (a.zero)
             for (int k = tid; k < a.nzero; k += NT) a.zero[k] = 0.0;
         if (a.zero1 && tid == 0) *a.zero1 = 0.0;
@@ -338,6 +338,11 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass_sym(SymRowArgs<
     if (tid == 0) out[size_t(C) * ld2] = cpx<T>{T(0), T(0)};   // padding column C
 }
 
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass_sym(SymRowArgs<T> a) {
+    rowpass_sym_body<T, LOG2N>(a, blockIdx.x);
+}
+
 template <typename T>
 struct SymColArgs {
     const T* Ts;
@@ -348,14 +353,14 @@ struct SymColArgs {
 };
 
 template <typename T, int LOG2N>
-__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass_sym(SymColArgs<T> a) {
+__device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk, int nblk) {
     using S = FftShape<LOG2N>;
     using F = BlockFft<T, LOG2N, false>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
     constexpr int NPAIR = (C + 1) / 2;
     __shared__ T lds[S::LDS_ELEMS];
-    const int pair = xcd_column(blockIdx.x, gridDim.x);
+    const int pair = xcd_column(blk, nblk);
     if (pair >= NPAIR) return;
     const int la = 2 * pair, lb = la + 1;
     const int tid = threadIdx.x;
@@ -377,6 +382,11 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass_sym(SymColArgs<
             if (has_b) hb[k] = v[r].im;
         }
     }
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass_sym(SymColArgs<T> a) {
+    colpass_sym_body<T, LOG2N>(a, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,9 +550,9 @@ __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict_
 }
 
 template <typename T, int MODE, bool SYM>
-__global__ __launch_bounds__(256) void k_sample(SampleArgs<T> a) {
+__device__ __forceinline__ void sample_body(const SampleArgs<T>& a, int blk) {
     __shared__ double red[4];
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = blk * 256 + threadIdx.x;
     double acc = 0.0;
     bool oob = false;
     if (i < a.nd) {
@@ -560,8 +570,13 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs<T> a) {
     if (oob && a.flag) atomicOr(a.flag, 1u);
     if constexpr (MODE == MODE_CHI2) {
         const double b = block_sum256(acc, red);
-        if (threadIdx.x == 0) a.partials[blockIdx.x] = b;
+        if (threadIdx.x == 0) a.partials[blk] = b;
     }
+}
+
+template <typename T, int MODE, bool SYM>
+__global__ __launch_bounds__(256) void k_sample(SampleArgs<T> a) {
+    sample_body<T, MODE, SYM>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -20,7 +20,9 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import check as _check, lib
+import weakref
+
+from ._lib import BATCH_EAGER, DatasetF32, DatasetF64, check as _check, lib
 
 _SUF = {torch.float64: 'f64', torch.float32: 'f32'}
 
@@ -241,3 +243,126 @@ def chi2_profile_autograd(dataset, intensity, geom, Rmin, dR, nxy, dxy, check_ce
     """Differentiable chi2Profile: intensity (nr) on the dataset's device, geom = tensor
     (inc, PA, dRA, dDec) in radians.  Returns a 0-dim float64 tensor."""
     return Chi2ProfileFunction.apply(intensity, geom, dataset, Rmin, dR, nxy, dxy, check_central_pixel)
+
+
+def _destroy_plan(handle: int) -> None:
+    lib.galario_batch_destroy(ctypes.c_void_p(handle))
+
+
+class BatchedChi2Profile:
+    """chi2Profile of many walkers' parameter sets over several datasets.
+
+    ``sets``: sequence of ``(dataset, nxy, dxy)`` with ``Dataset`` objects
+    holding observations (all in the same dtype, on the same device).
+    Datasets on the same grid share one spectrum per walker.  A call takes
+    ``intensity`` [W, nr] and ``geom`` [W, 4] = (inc, PA, dRA, dDec) device
+    tensors in the datasets' dtype and returns the per-walker chi2 summed over
+    the datasets (float64 [W]); ``return_sets=True`` also returns the
+    per-dataset values [W, S].  The first call for a given W times chunk
+    sizes (synchronising) and captures the launch sequence in a HIP graph;
+    later calls replay it on torch's current stream.  Points outside a grid
+    make the walker's chi2 NaN (``take_oob_flag``)."""
+
+    def __init__(self, sets, nr, Rmin, dR, max_walkers, check_central_pixel=True):
+        sets = list(sets)
+        if not sets:
+            raise ValueError('at least one dataset is needed')
+        ds0 = sets[0][0]
+        self.dtype, self.device = ds0.dtype, ds0.device
+        self.nr, self.nsets, self.max_walkers = int(nr), len(sets), int(max_walkers)
+        suf = _SUF[self.dtype]
+        St = DatasetF64 if suf == 'f64' else DatasetF32
+        arr = (St * len(sets))()
+        self._keep = []
+        for k, (ds, nxy, dxy) in enumerate(sets):
+            if not ds.has_obs:
+                raise ValueError('every dataset needs observed visibilities')
+            if ds.dtype != self.dtype or ds.device != self.device:
+                raise ValueError('all datasets must share dtype and device')
+            self._keep.append(ds)
+            arr[k] = St(int(nxy), dxy, 1.0 / (dxy * nxy), ds.n, ds.u.data_ptr(), ds.v.data_ptr(),
+                        ds.re.data_ptr(), ds.im.data_ptr(), ds.w.data_ptr())
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _check(getattr(lib, f'galario_batch_create_{suf}')(len(sets), arr, self.nr, Rmin, dR, self.max_walkers,
+                                                               int(bool(check_central_pixel)), ctypes.byref(h)))
+        self._h = h
+        self._fin = weakref.finalize(self, _destroy_plan, h.value)
+        self._run = getattr(lib, f'galario_batch_chi2_profile_{suf}')
+        self._run_grad = getattr(lib, f'galario_batch_chi2_profile_grad_{suf}')
+
+    def _inputs(self, intensity, geom):
+        _check_tensors(self.dtype, intensity, geom)
+        if intensity.device != self.device:
+            raise ValueError('parameters must be on the datasets\' device')
+        if intensity.dim() != 2 or intensity.shape[1] != self.nr:
+            raise ValueError(f'intensity must be [W, {self.nr}]')
+        W = intensity.shape[0]
+        if tuple(geom.shape) != (W, 4):
+            raise ValueError('geom must be [W, 4] = (inc, PA, dRA, dDec)')
+        return W
+
+    def __call__(self, intensity, geom, return_sets=False, eager=False):
+        W = self._inputs(intensity, geom)
+        chi2 = torch.empty(W, dtype=torch.float64, device=self.device)
+        per = torch.empty(W, self.nsets, dtype=torch.float64, device=self.device) if return_sets else None
+        with torch.cuda.device(self.device):
+            _check(self._run(self._h, _stream_ptr(self.device), W, _p(intensity), _p(geom),
+                             _p(per) if per is not None else None, _p(chi2), BATCH_EAGER if eager else 0))
+        return (chi2, per) if return_sets else chi2
+
+    def grad(self, intensity, geom, eager=False):
+        """(chi2 [W], chi2 per dataset [W, S], dchi2/dintensity [W, nr], dchi2/d(inc, PA, dRA, dDec) [W, 4])."""
+        W = self._inputs(intensity, geom)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        chi2, per = torch.empty(W, **f64), torch.empty(W, self.nsets, **f64)
+        gI, gg = torch.empty(W, self.nr, **f64), torch.empty(W, 4, **f64)
+        with torch.cuda.device(self.device):
+            _check(self._run_grad(self._h, _stream_ptr(self.device), W, _p(intensity), _p(geom), _p(per), _p(chi2),
+                                  _p(gI), _p(gg), BATCH_EAGER if eager else 0))
+        return chi2, per, gI, gg
+
+    def autograd(self, intensity, geom):
+        """Per-walker chi2 [W] differentiable with respect to intensity [W, nr] and geom [W, 4]."""
+        return _BatchedChi2Function.apply(self, intensity, geom)
+
+    def info(self, nwalkers, grad=False):
+        """(chunk size picked for nwalkers, whether its graph is captured)."""
+        c, g = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.galario_batch_info(self._h, int(nwalkers), int(bool(grad)), ctypes.byref(c), ctypes.byref(g)))
+        return c.value, bool(g.value)
+
+    def set_chunk(self, nwalkers, chunk, grad=False):
+        _check(lib.galario_batch_set_chunk(self._h, int(nwalkers), int(bool(grad)), int(chunk)))
+
+    def take_oob_flag(self) -> bool:
+        out = ctypes.c_int(0)
+        with torch.cuda.device(self.device):
+            _check(lib.galario_batch_take_oob_flag(self._h, ctypes.byref(out)))
+        return bool(out.value)
+
+    def close(self):
+        self._fin()
+
+
+class _BatchedChi2Function(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, plan, intensity, geom):
+        I = intensity.detach().contiguous()
+        g = geom.detach().to(I.device, I.dtype).contiguous()
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            chi2, _, gI, gg = plan.grad(I, g)
+            ctx.save_for_backward(gI, gg)
+        else:
+            chi2 = plan(I, g)
+        ctx.geom_device = geom.device
+        ctx.dtypes = (intensity.dtype, geom.dtype)
+        return chi2
+
+    @staticmethod
+    def backward(ctx, go):
+        gI, gg = ctx.saved_tensors
+        go = go.to(gI.dtype)[:, None]
+        dI = (go * gI).to(ctx.dtypes[0]) if ctx.needs_input_grad[1] else None
+        dg = (go * gg).to(ctx.geom_device, ctx.dtypes[1]) if ctx.needs_input_grad[2] else None
+        return None, dI, dg

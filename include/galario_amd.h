@@ -204,6 +204,72 @@ int galario_chi2_profile_grad_dev_f32(void *stream, int nr, const float *intensi
                                       const float *vis_obs_re, const float *vis_obs_im, const float *weights,
                                       int check_central, double *chi2_d, double *grad_I, double *grad_geom);
 
+/* ---- batched form (no counterpart in galario, whose callers loop over
+ *      chi2Profile per walker and dataset, docs/quickstart.rst:147-174) ---- */
+
+/* A dataset of the batched form: device arrays of nd points (the caller keeps
+ * them alive and unchanged while a plan uses them), its grid nxy (a power of
+ * two from 128 to 8192 in double, 16384 in float) and cell size dxy, and
+ * duv (galario passes 1/(nxy*dxy), python/libcommon.pyx:723). */
+typedef struct {
+    int nxy;
+    double dxy, duv;
+    int nd;
+    const double *u, *v, *vis_obs_re, *vis_obs_im, *weights;
+} galario_dataset_f64;
+typedef struct {
+    int nxy;
+    float dxy, duv;
+    int nd;
+    const float *u, *v, *vis_obs_re, *vis_obs_im, *weights;
+} galario_dataset_f32;
+
+/* Plan for up to max_walkers parameter sets, each a profile of nr values on
+ * (Rmin, dR) and (inc, PA, dRA, dDec), evaluated against nsets (<= 32)
+ * datasets on the current device.  Datasets on the same (nxy, dxy) share one
+ * spectrum per walker.  check_central as above (checked here, per grid). */
+int galario_batch_create_f64(int nsets, const galario_dataset_f64 *sets, int nr, double Rmin, double dR,
+                             int max_walkers, int check_central, void **plan);
+int galario_batch_create_f32(int nsets, const galario_dataset_f32 *sets, int nr, float Rmin, float dR,
+                             int max_walkers, int check_central, void **plan);
+int galario_batch_destroy(void *plan);
+
+#define GALARIO_BATCH_EAGER 1   /* flag: launch kernel by kernel instead of the captured graph */
+
+/* chi2 of nwalkers parameter sets: intensity [nwalkers][nr], geom
+ * [nwalkers][4] = (inc, PA, dRA, dDec), device arrays; outputs (device
+ * doubles, either may be null) chi2_sets [nwalkers][nsets] and chi2
+ * [nwalkers] = sum over the datasets.  Enqueued on `stream`.  The first call
+ * for a given nwalkers times chunk sizes (walkers whose spectra are built and
+ * sampled together) on `stream` and keeps the fastest (synchronising), then
+ * captures the launch sequence in a graph; later calls copy the parameters
+ * in, launch the graph and copy the results out.  chi2_sets[w][s] equals the
+ * single-call chi2 of walker w on dataset s (same kernels and summation
+ * order; the per-walker cos/sin are computed on the device).  A point outside
+ * a grid makes that chi2 NaN and sets the flag read by
+ * galario_batch_take_oob_flag.  Calls on one plan are ordered across streams
+ * by the library. */
+int galario_batch_chi2_profile_f64(void *plan, void *stream, int nwalkers, const double *intensity,
+                                   const double *geom, double *chi2_sets, double *chi2, int flags);
+int galario_batch_chi2_profile_f32(void *plan, void *stream, int nwalkers, const float *intensity,
+                                   const float *geom, double *chi2_sets, double *chi2, int flags);
+/* Same with the gradient: grad_I [nwalkers][nr] = dchi2/dintensity and
+ * grad_geom [nwalkers][4] = dchi2/d(inc, PA, dRA, dDec) of the sum over the
+ * datasets (device doubles, required). */
+int galario_batch_chi2_profile_grad_f64(void *plan, void *stream, int nwalkers, const double *intensity,
+                                        const double *geom, double *chi2_sets, double *chi2, double *grad_I,
+                                        double *grad_geom, int flags);
+int galario_batch_chi2_profile_grad_f32(void *plan, void *stream, int nwalkers, const float *intensity,
+                                        const float *geom, double *chi2_sets, double *chi2, double *grad_I,
+                                        double *grad_geom, int flags);
+/* The chunk size picked for (nwalkers, grad) (0 before the first call) and
+ * whether its graph is captured. */
+int galario_batch_info(void *plan, int nwalkers, int grad, int *chunk, int *graph_ready);
+/* Overrides the timed choice of the chunk size for (nwalkers, grad). */
+int galario_batch_set_chunk(void *plan, int nwalkers, int grad, int chunk);
+/* Synchronises the plan's last stream; returns (and clears) its out-of-grid flag. */
+int galario_batch_take_oob_flag(void *plan, int *out);
+
 /* Per-stage timing of the calling thread's _dev calls (no counterpart in
  * galario, whose GALARIO_TIMING build prints cudaEvent timings per stage,
  * src/galario.cpp:275-315): enable=1 starts recording a HIP event before each

@@ -127,3 +127,33 @@ def test_threads_setting():
     t0 = g.threads()
     assert g.threads(8) == 8 and g.threads() == 8
     g.threads(t0)
+
+
+def test_batch_plan_argument_checks_before_any_gpu_work():
+    """The batched form validates its datasets on the host (no GPU needed)."""
+    from galario_amd import _lib
+    a = galario_amd.arcsec
+    create = _lib.lib.galario_batch_create_f64
+    plan = ctypes.c_void_p()
+
+    def ds(nxy, dxy, nd=10, ptr=0x1000):
+        return _lib.DatasetF64(nxy, dxy, 1.0 / (nxy * dxy), nd, ptr, ptr, ptr, ptr, ptr)
+
+    def status(sets, nr=100, Rmin=1e-4 * a, dR=0.0005 * a, max_w=4, cc=0):
+        arr = (_lib.DatasetF64 * max(len(sets), 1))(*sets)
+        return create(len(sets), arr, nr, Rmin, dR, max_w, cc, ctypes.byref(plan))
+
+    with pytest.raises(ValueError, match="number of datasets"):
+        _lib.check(status([]))
+    with pytest.raises(ValueError, match="not supported by the batched form"):
+        _lib.check(status([ds(1000, 0.01 * a)]))
+    with pytest.raises(ValueError, match="is odd"):
+        _lib.check(status([ds(1023, 0.01 * a)]))
+    with pytest.raises(ValueError, match="null array"):
+        _lib.check(status([ds(512, 0.01 * a, ptr=0)]))
+    with pytest.raises(ValueError, match=r"Expect \(dxy/2-Rmin\)/dR > 5"):
+        _lib.check(status([ds(512, 0.01 * a)], dR=0.002 * a, cc=1))
+    with pytest.raises(ValueError, match="max_walkers"):
+        _lib.check(status([ds(512, 0.01 * a)], max_w=0))
+    with pytest.raises(ValueError, match="not a batch plan"):
+        _lib.check(_lib.lib.galario_batch_info(ctypes.c_void_p(0), 1, 0, None, None))
