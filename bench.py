@@ -78,6 +78,66 @@ def measured_traffic(kernel, nxy, nvis):
     return None
 
 
+def batched_rates(ta, ds, wl, I, chi2_plain, counts, counts_grad, steps, stream):
+    """Batched form over the same dataset: P parameter sets (walker 0 = the plain call's parameters, the
+    others jittered: profile x lognormal(0.1), inc, PA, dRA, dDec perturbed) per launch of the captured
+    graph; evals/s = P x graph launches/s, timed like the plain form."""
+    import numpy as np
+    import torch
+    counts = [int(c) for c in counts.split(',') if c.strip()]
+    counts_grad = [int(c) for c in counts_grad.split(',') if c.strip()]
+    pmax = max(counts + counts_grad)
+    rng = np.random.default_rng(7)
+    nr = len(wl.intensity)
+    Ib = np.repeat(wl.intensity[None, :], pmax, 0) * np.exp(0.1 * rng.standard_normal((pmax, 1)))
+    g0 = np.array([wl.inc, wl.PA, wl.dRA, wl.dDec])
+    gb = g0[None, :] + np.array([0.05, 0.1, 0.002 * 4.84813681109536e-06, 0.002 * 4.84813681109536e-06])[None, :] \
+        * rng.standard_normal((pmax, 4))
+    Ib[0], gb[0] = wl.intensity, g0
+    dev = I.device
+    It = torch.from_numpy(np.ascontiguousarray(Ib)).to(dev)
+    Gt = torch.from_numpy(np.ascontiguousarray(gb)).to(dev)
+    plan = ta.BatchedChi2Profile([(ds, wl.nxy, wl.dxy)], nr, wl.Rmin, wl.dR, max_walkers=pmax,
+                                 check_central_pixel=False)
+    out = {'workload': f'batched form: P parameter sets on the same dataset (nxy={wl.nxy}, Nvis={len(wl.u)}), '
+                       f'one captured HIP graph per call', 'forward': {}, 'forward_grad': {}}
+
+    def rate(fn, P, reps):
+        fn()                                                  # tuning + capture
+        torch.cuda.synchronize()
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        return ms
+
+    for P in counts:
+        Ip, Gp = It[:P].contiguous(), Gt[:P].contiguous()
+        reps = max(5, min(steps, int(2000 // P)))
+        ms = rate(lambda: plan(Ip, Gp), P, reps)
+        c = plan(Ip, Gp)
+        torch.cuda.synchronize()
+        rel = abs(float(c[0]) - chi2_plain) / abs(chi2_plain)
+        chunk, _ = plan.info(P)
+        out['forward'][str(P)] = {'evals_per_s': round(P * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4),
+                                  'chunk': chunk, 'chi2_rel_diff_vs_plain': rel}
+    for P in counts_grad:
+        Ip, Gp = It[:P].contiguous(), Gt[:P].contiguous()
+        reps = max(5, min(steps, int(1000 // P)))
+        ms = rate(lambda: plan.grad(Ip, Gp), P, reps)
+        chunk, _ = plan.info(P, grad=True)
+        out['forward_grad'][str(P)] = {'evals_per_s': round(P * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4),
+                                       'chunk': chunk}
+    plan.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -86,6 +146,10 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=12.0, help='CPU-baseline time budget (0: skip)')
     ap.add_argument('--nvis', type=int, default=1_000_000)
     ap.add_argument('--nxy', type=int, default=4096)
+    ap.add_argument('--batch', type=str, default='1,8,64,256',
+                    help='parameter-set counts of the batched form (comma list; empty: skip)')
+    ap.add_argument('--batch-grad', type=str, default='1,8,64',
+                    help='parameter-set counts of the batched forward+gradient')
     ap.add_argument('--copy-calibration-mb', type=int, default=0,
                     help='first run 5 device-to-device copies of this many MiB (HBM byte-counter calibration)')
     args = ap.parse_args()
@@ -209,6 +273,10 @@ def main():
     C = wl.nxy // 2 + 1
     b_complex = 2 * 8 * wl.nxy * C + 13 * 8 * len(wl.u) + 8 * len(wl.intensity)   # complex-spectrum model
 
+    batched = None
+    if args.batch:
+        batched = batched_rates(ta, ds, wl, I, chi2_dev, args.batch, args.batch_grad, args.steps, stream)
+
     # host-pointer (drop-in) form, PCIe uploads included: a few calls
     nh = 5
     t = time.perf_counter()
@@ -258,6 +326,7 @@ def main():
                          'alg_bytes_per_eval': b_alg_grad,
                          'frac': round(b_alg_grad * evals_grad / (HBM_PEAK_GBS * 1e9), 4),
                          'stages': stages_grad},
+        'batched': batched,
         'host_api_evals_per_s_incl_pcie': round(host_rate, 2),
         'cpu_baseline': cpu,
     }

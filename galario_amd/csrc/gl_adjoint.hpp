@@ -169,14 +169,34 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
     const int nwin = use_win ? wu * wm : 0;
     for (int j = tid; j < nwin; j += 256) win[j] = 0.0;
     __syncthreads();
-    if (live) {
-        const int cu[4] = {cu0, cu0, cu1, cu1}, cm[4] = {cm0, cm1, cm0, cm1};
+    if (use_win) {
+        if (live) {
+            const int cu[4] = {cu0, cu0, cu1, cu1}, cm[4] = {cm0, cm1, cm0, cm1};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (gk[k] == 0.0) continue;
-            if (cu[k] < GHOT && cm[k] < GHOT) lds_add_f64(&hot[cu[k] * GHOT + cm[k]], gk[k]);
-            else if (use_win) lds_add_f64(&win[(cu[k] - umin) * wm + (cm[k] - mmin)], gk[k]);
-            else atomic_add_f64(g.Hh + size_t(cu[k]) * a.ldH + cm[k], gk[k]);
+            for (int k = 0; k < 4; ++k) {
+                if (gk[k] == 0.0) continue;
+                if (cu[k] < GHOT && cm[k] < GHOT) lds_add_f64(&hot[cu[k] * GHOT + cm[k]], gk[k]);
+                else lds_add_f64(&win[(cu[k] - umin) * wm + (cm[k] - mmin)], gk[k]);
+            }
+        }
+    } else {
+        // Straight to Hh.  The FP64 atomics are served per 64-byte line and wave instruction, so lane
+        // pairs issue the two corners of one column of one point together (adjacent rows m0, m1): in
+        // round r, lane l carries point (l >> 1) + 32 (r & 1) of its wave, column r >> 1, row m0 or m1.
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int src = (lane >> 1) + 32 * (r & 1);
+            const bool hi = lane & 1;
+            const int col = __shfl((r >> 1) ? cu1 : cu0, src, 64);
+            const int m_a = __shfl(cm0, src, 64), m_b = __shfl(cm1, src, 64);
+            const double v_a = __shfl((r >> 1) ? gk[2] : gk[0], src, 64);
+            const double v_b = __shfl((r >> 1) ? gk[3] : gk[1], src, 64);
+            const int m = hi ? m_b : m_a;
+            const double x = hi ? v_b : v_a;
+            if (x != 0.0) {
+                if (col < GHOT && m < GHOT) lds_add_f64(&hot[col * GHOT + m], x);
+                else atomic_add_f64(g.Hh + size_t(col) * a.ldH + m, x);
+            }
         }
     }
     __syncthreads();
