@@ -124,14 +124,14 @@ def batched_rates(ta, ds, wl, I, chi2_plain, counts, counts_grad, steps, stream)
         c = plan(Ip, Gp)
         torch.cuda.synchronize()
         rel = abs(float(c[0]) - chi2_plain) / abs(chi2_plain)
-        chunk, _ = plan.info(P)
+        chunk, kw, _ = plan.info(P)
         out['forward'][str(P)] = {'evals_per_s': round(P * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4),
-                                  'chunk': chunk, 'chi2_rel_diff_vs_plain': rel}
+                                  'chunk': chunk, 'walkers_per_sample': kw, 'chi2_rel_diff_vs_plain': rel}
     for P in counts_grad:
         Ip, Gp = It[:P].contiguous(), Gt[:P].contiguous()
         reps = max(5, min(steps, int(1000 // P)))
         ms = rate(lambda: plan.grad(Ip, Gp), P, reps)
-        chunk, _ = plan.info(P, grad=True)
+        chunk, _, _ = plan.info(P, grad=True)
         out['forward_grad'][str(P)] = {'evals_per_s': round(P * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4),
                                        'chunk': chunk}
     plan.close()

@@ -74,8 +74,8 @@ def _load():
     lib.galario_stage_repeat.argtypes = [_c_int, _c_int]
     lib.galario_stage_times.argtypes = [ctypes.POINTER(ctypes.c_double), _c_int, _ip]
     lib.galario_batch_destroy.argtypes = [_vp]
-    lib.galario_batch_info.argtypes = [_vp, _c_int, _c_int, _ip, _ip]
-    lib.galario_batch_set_chunk.argtypes = [_vp, _c_int, _c_int, _c_int]
+    lib.galario_batch_info.argtypes = [_vp, _c_int, _c_int, _ip, _ip, _ip]
+    lib.galario_batch_set_chunk.argtypes = [_vp, _c_int, _c_int, _c_int, _c_int]
     lib.galario_batch_take_oob_flag.argtypes = [_vp, _ip]
     for suf, real, ds in (('f64', ctypes.c_double, DatasetF64), ('f32', ctypes.c_float, DatasetF32)):
         R = real

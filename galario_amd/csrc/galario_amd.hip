@@ -1054,6 +1054,7 @@ struct BatchPlan : BatchPlanBase {
     hipEvent_t ev = nullptr;
     struct Exec {
         int chunk = 0;
+        int kw = 4;                    // walkers per sample workgroup (forward)
         hipGraphExec_t exec = nullptr;
     };
     std::map<std::pair<int, int>, Exec> runs;   // (nwalkers, grad)
@@ -1142,7 +1143,18 @@ BatchPtrs<T> batch_ptrs(BatchPlan<T>& P, bool grad) {
 // Enqueue one batched evaluation of W walkers (parameters already in P.I /
 // P.geom) in chunks of `chunk` walkers; only kernel launches (capturable).
 template <typename T>
-void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad) {
+void launch_sample_multi(int kw, dim3 grid, hipStream_t s, const SampleArgs<T>& a, const BatchPtrs<T>& b, int nw) {
+    switch (kw) {
+        case 1: hipLaunchKernelGGL((k_sample_multi_b<T, 1>), grid, dim3(256), 0, s, a, b, nw); break;
+        case 2: hipLaunchKernelGGL((k_sample_multi_b<T, 2>), grid, dim3(256), 0, s, a, b, nw); break;
+        case 4: hipLaunchKernelGGL((k_sample_multi_b<T, 4>), grid, dim3(256), 0, s, a, b, nw); break;
+        case 8: hipLaunchKernelGGL((k_sample_multi_b<T, 8>), grid, dim3(256), 0, s, a, b, nw); break;
+        default: fail(GALARIO_ERR_RUNTIME, "internal: walkers per sample workgroup must be 1, 2, 4 or 8");
+    }
+}
+
+template <typename T>
+void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, int kw) {
     const int nr = P.nr;
     const size_t nz = std::max(size_t(W), grad ? size_t(W) * size_t(std::max(nr, 4)) : size_t(W));
     hipLaunchKernelGGL((k_batch_setup<T>), dim3(blocks_for(nz, 256)), dim3(256), 0, s, W,
@@ -1186,7 +1198,7 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad) 
                 if (grad)
                     hipLaunchKernelGGL((k_sample_grad_sym_b<T>), dim3(P.nb[si], nw), dim3(256), 0, s, a, b);
                 else
-                    hipLaunchKernelGGL((k_sample_b<T>), dim3(P.nb[si], nw), dim3(256), 0, s, a, b);
+                    launch_sample_multi<T>(kw, dim3(P.nb[si], (nw + kw - 1) / kw), s, a, b, nw);
             }
             if (grad) {
                 AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0};
@@ -1311,15 +1323,15 @@ BatchPlan<T>& plan_of(void* h) {
 
 // Average time of one eager evaluation at `chunk` (after one warm-up).
 template <typename T>
-float batch_time(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad) {
+float batch_time(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, int kw) {
     P.ensure_slots(chunk, grad, s);
-    batch_enqueue(P, s, W, chunk, grad);
+    batch_enqueue(P, s, W, chunk, grad, kw);
     hipEvent_t e0, e1;
     HC(hipEventCreate(&e0));
     HC(hipEventCreate(&e1));
     HC(hipEventRecord(e0, s));
     const int reps = 2;
-    for (int r = 0; r < reps; ++r) batch_enqueue(P, s, W, chunk, grad);
+    for (int r = 0; r < reps; ++r) batch_enqueue(P, s, W, chunk, grad, kw);
     HC(hipEventRecord(e1, s));
     HC(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1329,24 +1341,35 @@ float batch_time(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad) {
     return ms / reps;
 }
 
-// Chunk size for W walkers: the fastest power of two (and W itself) within
-// the slot budget, by timing eager evaluations on the caller's stream.
+// Launch shape for W walkers, by timing eager evaluations on the caller's
+// stream: the chunk size (powers of two and W itself, within the slot
+// budget) at 4 walkers per sample workgroup, then (forward) the walkers per
+// sample workgroup (1, 2, 4, 8) at that chunk size.
 template <typename T>
-int batch_tune(BatchPlan<T>& P, hipStream_t s, int W, bool grad) {
+void batch_tune(BatchPlan<T>& P, hipStream_t s, int W, bool grad, int* chunk, int* kw) {
     const int cmax = P.max_chunk(grad, W);
     std::vector<int> cand;
     for (int c = 1; c < cmax; c *= 2) cand.push_back(c);
     cand.push_back(cmax);
-    int best = cand.back();
+    int best = cand.back(), bkw = 4;
     float tbest = 0.f;
     for (int c : cand) {
-        const float t = batch_time(P, s, W, c, grad);
+        const float t = batch_time(P, s, W, c, grad, 4);
         if (c == cand.front() || t < tbest) {
             tbest = t;
             best = c;
         }
     }
-    return best;
+    if (!grad)
+        for (int k : {1, 2, 8}) {
+            const float t = batch_time(P, s, W, best, grad, k);
+            if (t < tbest) {
+                tbest = t;
+                bkw = k;
+            }
+        }
+    *chunk = best;
+    *kw = bkw;
 }
 
 template <typename T>
@@ -1372,10 +1395,10 @@ int batch_run(void* h, void* stream, int W, const T* I, const T* geom, double* c
             HC(hipMemcpyAsync(P.geom.p, geom, sizeof(T) * size_t(W) * 4, hipMemcpyDeviceToDevice, s));
         const auto key = std::make_pair(W, grad ? 1 : 0);
         auto& run = P.runs[key];
-        if (run.chunk == 0) run.chunk = batch_tune(P, s, W, grad);
+        if (run.chunk == 0) batch_tune(P, s, W, grad, &run.chunk, &run.kw);
         P.ensure_slots(run.chunk, grad, s);
         if (flags & GALARIO_BATCH_EAGER) {
-            batch_enqueue(P, s, W, run.chunk, grad);
+            batch_enqueue(P, s, W, run.chunk, grad, run.kw);
         } else {
             if (!run.exec) {
                 // capture on the plan's stream (nothing runs during capture)
@@ -1383,7 +1406,7 @@ int batch_run(void* h, void* stream, int W, const T* I, const T* geom, double* c
                 hipGraph_t graph = nullptr;
                 HC(hipStreamBeginCapture(P.cap, hipStreamCaptureModeThreadLocal));
                 try {
-                    batch_enqueue(P, P.cap, W, run.chunk, grad);
+                    batch_enqueue(P, P.cap, W, run.chunk, grad, run.kw);
                 } catch (...) {
                     (void)hipStreamEndCapture(P.cap, &graph);
                     if (graph) (void)hipGraphDestroy(graph);
@@ -1565,27 +1588,29 @@ int galario_batch_destroy(void* plan) {
     });
 }
 
-int galario_batch_info(void* plan, int nwalkers, int grad, int* chunk, int* graph_ready) {
+int galario_batch_info(void* plan, int nwalkers, int grad, int* chunk, int* walkers_per_sample, int* graph_ready) {
     return guard([&] {
         auto* b = static_cast<BatchPlanBase*>(plan);
         if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
-        int c = 0, g = 0;
+        int c = 0, k = 0, g = 0;
         auto look = [&](auto& P) {
             std::lock_guard<std::mutex> lk(P.mu);
             auto it = P.runs.find(std::make_pair(nwalkers, grad ? 1 : 0));
             if (it != P.runs.end()) {
                 c = it->second.chunk;
+                k = grad ? 1 : it->second.kw;
                 g = it->second.exec != nullptr;
             }
         };
         if (b->prec == 8) look(*static_cast<BatchPlan<double>*>(b));
         else look(*static_cast<BatchPlan<float>*>(b));
         if (chunk) *chunk = c;
+        if (walkers_per_sample) *walkers_per_sample = k;
         if (graph_ready) *graph_ready = g;
     });
 }
 
-int galario_batch_set_chunk(void* plan, int nwalkers, int grad, int chunk) {
+int galario_batch_set_chunk(void* plan, int nwalkers, int grad, int chunk, int walkers_per_sample) {
     return guard([&] {
         auto* b = static_cast<BatchPlanBase*>(plan);
         if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
@@ -1595,10 +1620,14 @@ int galario_batch_set_chunk(void* plan, int nwalkers, int grad, int chunk) {
             if (chunk < 1 || chunk > P.max_chunk(grad != 0, nwalkers))
                 invalid("batch", "chunk = " + std::to_string(chunk) + " (expect 1 to " +
                                      std::to_string(P.max_chunk(grad != 0, nwalkers)) + ")");
+            if (walkers_per_sample != 1 && walkers_per_sample != 2 && walkers_per_sample != 4 &&
+                walkers_per_sample != 8)
+                invalid("batch", "walkers per sample workgroup must be 1, 2, 4 or 8");
             auto& run = P.runs[std::make_pair(nwalkers, grad ? 1 : 0)];
             if (run.exec) (void)hipGraphExecDestroy(run.exec);
             run.exec = nullptr;
             run.chunk = chunk;
+            run.kw = walkers_per_sample;
         };
         if (b->prec == 8) set(*static_cast<BatchPlan<double>*>(b));
         else set(*static_cast<BatchPlan<float>*>(b));

@@ -117,12 +117,51 @@ __device__ __forceinline__ void walker_sample_args(SampleArgs<T>& a, const Batch
     a.dDec2 = p.dDec2;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_sample_b(SampleArgs<T> a, BatchPtrs<T> b) {
-    const int slot = blockIdx.y, w = b.w0 + slot;
-    walker_sample_args(a, b, slot, w);
-    a.partials = b.part + size_t(w) * 4 * b.ldPart + b.partOff;
-    sample_body<T, MODE_CHI2, true>(a, blockIdx.x);
+// KW walkers per workgroup on the same points: the dataset streams (u, v, Re,
+// Im, w) are read once for KW walkers.  Each walker's per-workgroup partial
+// is the same fixed-order sum as in the one-walker kernel.
+template <typename T, int KW>
+__global__ __launch_bounds__(256) void k_sample_multi_b(SampleArgs<T> a, BatchPtrs<T> b, int nw) {
+    __shared__ double red[KW][4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int s0 = blockIdx.y * KW;
+    double acc[KW];
+    bool oob_any = false;
+#pragma unroll
+    for (int k = 0; k < KW; ++k) acc[k] = 0.0;
+    if (i < a.nd) {
+        const T u0 = a.u[i], v0 = a.v[i];
+        const T re = a.re[i], im = a.im[i];
+        const T sw = sqrt(a.w[i]);
+#pragma unroll
+        for (int k = 0; k < KW; ++k) {
+            const int slot = s0 + k;
+            if (slot < nw) {
+                SampleArgs<T> ak = a;
+                walker_sample_args(ak, b, slot, b.w0 + slot);
+                bool oob = false;
+                const cpx<T> V = sample_point_sym(ak, u0, v0, oob);
+                oob_any |= oob;
+                const T cr = (V.re - re) * sw;
+                const T ci = (V.im - im) * sw;
+                if constexpr (std::is_same<T, double>::value) acc[k] = cr * cr + ci * ci;
+                else acc[k] = double(cr) * cr + double(ci) * ci;
+            }
+        }
+    }
+    if (oob_any && a.flag) atomicOr(a.flag, 1u);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+        const double x = wave_sum(acc[k]);
+        if (lane == 0) red[k][wv] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < KW && s0 + int(threadIdx.x) < nw) {
+        const int k = threadIdx.x;
+        b.part[size_t(b.w0 + s0 + k) * 4 * b.ldPart + b.partOff + blockIdx.x] =
+            (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+    }
 }
 
 template <typename T>

@@ -327,13 +327,16 @@ class BatchedChi2Profile:
         return _BatchedChi2Function.apply(self, intensity, geom)
 
     def info(self, nwalkers, grad=False):
-        """(chunk size picked for nwalkers, whether its graph is captured)."""
-        c, g = ctypes.c_int(0), ctypes.c_int(0)
-        _check(lib.galario_batch_info(self._h, int(nwalkers), int(bool(grad)), ctypes.byref(c), ctypes.byref(g)))
-        return c.value, bool(g.value)
+        """(chunk size, walkers per sample workgroup) picked for nwalkers, and whether its graph is captured."""
+        c, k, g = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.galario_batch_info(self._h, int(nwalkers), int(bool(grad)), ctypes.byref(c), ctypes.byref(k),
+                                      ctypes.byref(g)))
+        return c.value, k.value, bool(g.value)
 
-    def set_chunk(self, nwalkers, chunk, grad=False):
-        _check(lib.galario_batch_set_chunk(self._h, int(nwalkers), int(bool(grad)), int(chunk)))
+    def set_chunk(self, nwalkers, chunk, walkers_per_sample=4, grad=False):
+        """Override the timed launch shape for nwalkers."""
+        _check(lib.galario_batch_set_chunk(self._h, int(nwalkers), int(bool(grad)), int(chunk),
+                                           int(walkers_per_sample)))
 
     def take_oob_flag(self) -> bool:
         out = ctypes.c_int(0)

@@ -262,11 +262,15 @@ int galario_batch_chi2_profile_grad_f64(void *plan, void *stream, int nwalkers, 
 int galario_batch_chi2_profile_grad_f32(void *plan, void *stream, int nwalkers, const float *intensity,
                                         const float *geom, double *chi2_sets, double *chi2, double *grad_I,
                                         double *grad_geom, int flags);
-/* The chunk size picked for (nwalkers, grad) (0 before the first call) and
- * whether its graph is captured. */
-int galario_batch_info(void *plan, int nwalkers, int grad, int *chunk, int *graph_ready);
-/* Overrides the timed choice of the chunk size for (nwalkers, grad). */
-int galario_batch_set_chunk(void *plan, int nwalkers, int grad, int chunk);
+/* The launch shape picked for (nwalkers, grad): chunk size (0 before the
+ * first call) and walkers per sample workgroup (the forward sample reads each
+ * dataset point once for that many walkers; 1 with the gradient), and
+ * whether its graph is captured.  Any pointer may be null. */
+int galario_batch_info(void *plan, int nwalkers, int grad, int *chunk, int *walkers_per_sample,
+                       int *graph_ready);
+/* Overrides the timed choice for (nwalkers, grad): chunk size and walkers
+ * per sample workgroup (1, 2, 4 or 8). */
+int galario_batch_set_chunk(void *plan, int nwalkers, int grad, int chunk, int walkers_per_sample);
 /* Synchronises the plan's last stream; returns (and clears) its out-of-grid flag. */
 int galario_batch_take_oob_flag(void *plan, int *out);
 

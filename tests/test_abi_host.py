@@ -156,4 +156,4 @@ def test_batch_plan_argument_checks_before_any_gpu_work():
     with pytest.raises(ValueError, match="max_walkers"):
         _lib.check(status([ds(512, 0.01 * a)], max_w=0))
     with pytest.raises(ValueError, match="not a batch plan"):
-        _lib.check(_lib.lib.galario_batch_info(ctypes.c_void_p(0), 1, 0, None, None))
+        _lib.check(_lib.lib.galario_batch_info(ctypes.c_void_p(0), 1, 0, None, None, None))

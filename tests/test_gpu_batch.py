@@ -77,11 +77,11 @@ def test_batch_forward_matches_single_calls_and_oracle():
     I, geom, Rmin, dR = walkers(W)
     plan = ta.BatchedChi2Profile(sets, I.shape[1], Rmin, dR, max_walkers=8, check_central_pixel=False)
     chi2, per = plan(I, geom, return_sets=True)
-    chunk, ready = plan.info(W)
-    assert chunk >= 1 and ready
+    chunk, kw, ready = plan.info(W)
+    assert chunk >= 1 and kw in (1, 2, 4, 8) and ready
     ref = np.array(plain(sets, I, geom, Rmin, dR))
     got = per.cpu().numpy()
-    print(f"\nchunk {chunk}; max rel diff vs single calls {np.max(np.abs(got - ref) / np.abs(ref)):.2e}")
+    print(f"\nchunk {chunk} x {kw}; max rel diff vs single calls {np.max(np.abs(got - ref) / np.abs(ref)):.2e}")
     np.testing.assert_allclose(got, ref, rtol=1e-12)
     np.testing.assert_allclose(chi2.cpu().numpy(), got.sum(axis=1), rtol=1e-15)
     # oracle: walker 2 on dataset 0
@@ -95,9 +95,9 @@ def test_batch_forward_matches_single_calls_and_oracle():
     c_graph = plan(I2, geom2).cpu().numpy()
     c_eager = plan(I2, geom2, eager=True).cpu().numpy()
     assert np.array_equal(c_graph, c_eager)
-    for c in (1, 3, W):
-        plan.set_chunk(W, c)
-        assert np.array_equal(plan(I2, geom2).cpu().numpy(), c_graph)
+    for c, kw in ((1, 1), (3, 2), (W, 4), (W, 8), (5, 8)):
+        plan.set_chunk(W, c, kw)
+        assert np.array_equal(plan(I2, geom2).cpu().numpy(), c_graph), (c, kw)
     ref2 = np.array(plain(sets, I2, geom2, Rmin, dR)).sum(axis=1)
     np.testing.assert_allclose(c_graph, ref2, rtol=1e-12)
     # fewer walkers than the plan holds: its own tuning and graph
