@@ -305,7 +305,7 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
             if (has_b) xb += hot_sum(lb, p);
         }
         return cpx<T>{T(wk * xa), T(wk * xb)};
-    });
+    }, a.Rs < NT);        // outputs m <= Rs' only
     // all reads of Hh are done (the FFT synchronised after its first pass): clear it for the next call
     for (int p = tid; p <= N / 2; p += NT) {
         ha[p] = 0.0;
@@ -412,7 +412,7 @@ __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk)
         const int l = k <= N / 2 ? k : N - k;
         const T wk = T(ext_weight(k, N));
         return cpx<T>{wk * ra[l], hb ? wk * rb[l] : T(0)};
-    });
+    }, a.Rs < NT);        // pixels j <= Rs' only
     double ginc = 0.0;
 #pragma unroll
     for (int r = 0; r < E; ++r) {

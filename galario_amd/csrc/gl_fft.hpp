@@ -166,8 +166,7 @@ template <typename T, int LOG2N, bool INV>
 struct BlockFft {
     using S = FftShape<LOG2N>;
     static constexpr int N = S::N, E = S::E, NT = S::NT;
-    static constexpr int TWC = 8;           // preloaded twiddle bases per pass and thread
-    using TwCache = cpx<T>[S::NPASS][TWC];
+    static constexpr int TWC = 8;           // twiddle bases per pass and thread
 
     static __device__ __forceinline__ cpx<T> twid(const cpx<T>* __restrict__ tw, int e) {
         const cpx<T> w = tw[e];
@@ -176,7 +175,8 @@ struct BlockFft {
 
     // Table reads of pass P: for each of the thread's E/R butterflies the bases
     // W^e1, W^2e1, W^4e1, W^8e1 (as many as log2 R).  They depend only on the
-    // thread index, so they are issued at kernel start, next to the data loads.
+    // thread index; they are issued before the LDS exchange that precedes the
+    // pass, so the exchange covers their latency.
     template <int P>
     static __device__ __forceinline__ void preload(cpx<T> (&tc)[TWC], int tid, const cpx<T>* __restrict__ tw) {
         constexpr int LR = S::log2radix(P);
@@ -269,13 +269,29 @@ struct BlockFft {
         }
     }
 
+    // Output 0 of the last pass only: X[tid] = sum_r v[r] W^{r e1}, by Horner's
+    // rule on the base twiddle W^{e1} (the last pass is radix E, one butterfly
+    // per thread).
+    template <int P>
+    static __device__ __forceinline__ void last_output0(cpx<T> (&v)[E], const cpx<T> (&tc)[TWC]) {
+        const cpx<T> w = tc[0];
+        cpx<T> acc = v[E - 1];
+#pragma unroll
+        for (int r = E - 2; r >= 0; --r) acc = cadd(cmul(acc, w), v[r]);
+        v[0] = acc;
+    }
+
     // Passes 1..NPASS-1 after the first pass has been computed in registers:
     // exchange through LDS (real parts, then imaginary parts, so the buffer
     // holds N reals), twiddle, DFT.  On return thread tid holds X[tid + r*N/E]
-    // in v[r] (the last pass is radix E with one butterfly per thread).
+    // in v[r] (the last pass is radix E with one butterfly per thread); with
+    // only0, v[0] = X[tid] alone (outputs below N/E).
     template <int P = 1>
-    static __device__ __forceinline__ void rest(cpx<T> (&v)[E], T* lds, int tid, const TwCache& tc) {
+    static __device__ __forceinline__ void rest(cpx<T> (&v)[E], T* lds, int tid, const cpx<T>* __restrict__ tw,
+                                                bool only0) {
         if constexpr (P < S::NPASS) {
+            cpx<T> tc[TWC];
+            preload<P>(tc, tid, tw);          // issued before the exchange, used after it
             store_lds<P - 1, 0>(lds, v, tid);
             __syncthreads();
             load_lds<P, 0>(v, lds, tid);
@@ -284,25 +300,27 @@ struct BlockFft {
             __syncthreads();
             load_lds<P, 1>(v, lds, tid);
             __syncthreads();
-            compute<P>(v, tc[P]);
-            rest<P + 1>(v, lds, tid, tc);
+            if constexpr (P == S::NPASS - 1) {
+                if (only0) {
+                    last_output0<P>(v, tc);
+                    return;
+                }
+            }
+            compute<P>(v, tc);
+            rest<P + 1>(v, lds, tid, tw, only0);
         }
     }
 
     // Full transform.  load(n) returns input element n; it is called only for
     // n in [0, Rs) U [N-Rs, N) (zeros elsewhere).  lds: LDS_ELEMS reals.  On
-    // return v[r] holds X[tid + r*N/E], and every thread has finished reading LDS.
+    // return v[r] holds X[tid + r*N/E] (only0: v[0] = X[tid] only), and every
+    // thread has finished reading LDS.
     template <typename Loader>
     static __device__ __forceinline__ void run(cpx<T> (&v)[E], T* lds, int tid, int Rs,
-                                               const cpx<T>* __restrict__ tw, Loader&& load) {
+                                               const cpx<T>* __restrict__ tw, Loader&& load, bool only0 = false) {
         constexpr int R0 = E;            // first pass is radix E, one butterfly per thread
         constexpr int STRIDE = N / R0;   // == NT
         const int j = tid;
-        TwCache tc;
-        static_for<1, S::NPASS>([&](auto pc) {
-            constexpr int P = decltype(pc)::value;
-            preload<P>(tc[P], tid, tw);
-        });
         if (Rs <= STRIDE) {
             // at most x[j] (r = 0) and x[j + (R0-1) N/R0] (r = R0-1) are non-zero
             const cpx<T> z = {T(0), T(0)};
@@ -318,7 +336,7 @@ struct BlockFft {
             }
             dft<R0, INV>(v);
         }
-        rest<1>(v, lds, tid, tc);
+        rest<1>(v, lds, tid, tw, only0);
     }
 };
 
