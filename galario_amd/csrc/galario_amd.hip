@@ -422,7 +422,7 @@ bool build_spectrum_sym(Workspace& w, hipStream_t s, const SweepPrm<T>& sw, int 
     t_timer.mark(ST_ROW, s);
     for (int r = t_timer.reps(ST_ROW); r > 0; --r) launch_rowpass_sym<T>(L, g.pairs_r, s, ra);
     t_timer.mark(ST_COL, s);
-    for (int r = t_timer.reps(ST_COL); r > 0; --r) launch_colpass_sym<T>(L, 8 * ((g.pairs_c + 7) / 8), s, ca);
+    for (int r = t_timer.reps(ST_COL); r > 0; --r) launch_colpass_sym<T>(L, N / 4, s, ca);   // N/4 pairs + column N/2
     HC(hipGetLastError());
     *H_out = H;
     *ldH_out = g.ldH;
@@ -1172,7 +1172,8 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, 
             const cpx<T>* tw = twiddles<T>(G.N);
             SymRowArgs<T> ra{G.sw, G.N, g.Rs, g.nrows, g.ldT, tw, nullptr, nullptr, 0, nullptr};
             SymColArgs<T> ca{nullptr, G.N, g.Rs, g.ldT, tw, nullptr, g.ldH};
-            const int cblocks = 8 * ((g.pairs_c + 7) / 8);
+            const int cblocks = G.N / 4;                      // column pairs 0..N/2-1, column N/2 inside
+            const int ablocks = 8 * ((g.pairs_c + 7) / 8);    // adjoint: (C+1)/2 pairs
 #define GL_CALL(LL)                                                                                              \
     hipLaunchKernelGGL((k_rowpass_sym_b<T, LL>), dim3(g.pairs_r, nw), dim3(FftShape<LL>::NT), 0, s, ra, b);      \
     hipLaunchKernelGGL((k_colpass_sym_b<T, LL>), dim3(cblocks, nw), dim3(FftShape<LL>::NT), 0, s, ca, b)
@@ -1205,7 +1206,7 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, 
                 aca.ldTh = int(P.sTh / size_t(g.nrows));
                 AdjRowArgs<T> ara{G.sw, nullptr, aca.ldTh, G.N, g.Rs, g.nrows, tw, 0.0, nullptr, nullptr};
 #define GL_CALL(LL)                                                                                              \
-    hipLaunchKernelGGL((k_adjcol_sym_b<T, LL>), dim3(cblocks, nw), dim3(FftShape<LL>::NT), 0, s, aca, b);        \
+    hipLaunchKernelGGL((k_adjcol_sym_b<T, LL>), dim3(ablocks, nw), dim3(FftShape<LL>::NT), 0, s, aca, b);        \
     hipLaunchKernelGGL((k_adjrow_sym_b<T, LL>), dim3(g.pairs_r, nw), dim3(FftShape<LL>::NT), 0, s, ara, b)
                 GL_L_SWITCH(T, GL_CALL)
 #undef GL_CALL

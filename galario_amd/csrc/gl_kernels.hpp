@@ -352,13 +352,35 @@ struct SymColArgs {
     int ldH;
 };
 
+// Column N/2 of H (the one column without a partner among the N/2 + 1):
+// H[N/2][k] = sum over the loaded rows n of Ts[|n|][N/2] cos(2 pi n k / N), i.e.
+// Ts[0] + 2 sum_{0<m<Rs} Ts[m] cos(2 pi m k/N) + Ts[Rs] cos(2 pi Rs k/N), as a
+// direct sum (FP64 accumulation) by one wave per output k = blk + q N/4.
+template <typename T, int LOG2N>
+__device__ __forceinline__ void colpass_nyquist(const SymColArgs<T>& a, int blk) {
+    constexpr int N = 1 << LOG2N, NT = FftShape<LOG2N>::NT;
+    constexpr int NW = NT / 64 > 0 ? NT / 64 : 1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const T* col = a.Ts + size_t(N / 2) * a.ldT;
+    for (int q = wv; q < 3; q += NW) {
+        const int k = blk + q * (N / 4);
+        if (k > N / 2) break;
+        double acc = 0.0;
+        for (int m = lane; m <= a.Rs; m += 64) {
+            const double wm = (m == 0 || m == a.Rs) ? 1.0 : 2.0;
+            acc += wm * double(col[m]) * double(a.tw[(m * k) & (N - 1)].re);
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) a.H[size_t(N / 2) * a.ldH + k] = T(acc);
+    }
+}
+
 template <typename T, int LOG2N>
 __device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk, int nblk) {
     using S = FftShape<LOG2N>;
     using F = BlockFft<T, LOG2N, false>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
-    constexpr int C = N / 2 + 1;
-    constexpr int NPAIR = (C + 1) / 2;
+    constexpr int NPAIR = N / 4;                  // pairs of columns 0..N/2-1; column N/2 separately
     __shared__ T lds[S::LDS_ELEMS];
     const int pair = xcd_column(blk, nblk);
     if (pair >= NPAIR) return;
@@ -373,15 +395,15 @@ __device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk
     });
     T* ha = a.H + size_t(la) * a.ldH;
     T* hb = ha + a.ldH;
-    const bool has_b = lb < C;
 #pragma unroll
     for (int r = 0; r <= E / 2; ++r) {
         const int k = tid + r * NT;
         if (k <= N / 2) {
             ha[k] = v[r].re;
-            if (has_b) hb[k] = v[r].im;
+            hb[k] = v[r].im;
         }
     }
+    colpass_nyquist<T, LOG2N>(a, pair);
 }
 
 template <typename T, int LOG2N>
