@@ -138,6 +138,126 @@ def batched_rates(ta, ds, wl, I, chi2_plain, counts, counts_grad, steps, stream)
     return out
 
 
+def extra_configs(ta, g, stream, steps):
+    """The other configurations of the benchmark suite (each timed like the main one, fewer steps):
+    differentiation at nxy=2048 (plain gradient, FP64 and FP32), the batched forward of 256 walkers over
+    three datasets of uneven nxy and Nvis, and the batched forward+gradient of 64 sets over three nxy=4096
+    datasets (FP64 and FP32, errors against the FP64 plain path), plus the FP32 plain forward."""
+    import numpy as np
+    import torch
+    from galario_amd import synthetic as syn
+    A = syn.ARCSEC
+    dev = torch.device('cuda:0')
+    f64 = torch.float64
+
+    def model(wl):
+        return g.sampleProfile(wl.intensity, wl.Rmin, wl.dR, wl.nxy, wl.dxy, wl.u, wl.v, check_central_pixel=False,
+                               **wl.kwargs())
+
+    def timeit(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    # differentiation: nxy=2048, Nvis=2e6, Nr=500, dR=0.004", 16 parameter points
+    wl = syn.ring_profile_workload(model, nvis=2_000_000, nxy=2048, nr=500, seed=2, dR=0.004 * A)
+    I16, G16 = syn.walker_ensemble(np.random.default_rng(2), 16, 500, wl.Rmin, wl.dR, wl.inc, wl.PA, wl.dRA,
+                                   wl.dDec)
+    res = {'workload': 'plain chi2Profile + gradient, nxy=2048, Nvis=2e6, Nr=500, 16 parameter points'}
+    grads = {}
+    for name, dt in (('f64', torch.float64), ('f32', torch.float32)):
+        ds = ta.Dataset(wl.u, wl.v, wl.re, wl.im, wl.w, device=dev, dtype=dt)
+        It = torch.from_numpy(I16).to(dev, dt)
+        calls = [dict(inc=float(G16[k, 0]), PA=float(G16[k, 1]), dRA=float(G16[k, 2]), dDec=float(G16[k, 3]))
+                 for k in range(16)]
+        it = iter(range(10 ** 9))
+
+        def one():
+            k = next(it) % 16
+            return ds.chi2_profile_grad(It[k], wl.Rmin, wl.dR, wl.nxy, wl.dxy, check_central_pixel=False, **calls[k])
+
+        ms = timeit(one, max(16, min(steps, 64)))
+        grads[name] = [one() for _ in range(16)]
+        res[name] = {'fwd_grad_evals_per_s': round(1000.0 / ms, 1), 'ms_per_eval': round(ms, 4)}
+        del ds
+    rel = max(float(torch.linalg.norm(grads['f32'][k][1] - grads['f64'][k][1]) / torch.linalg.norm(grads['f64'][k][1]))
+              for k in range(16))
+    res['f32_grad_I_rel_err_vs_f64'] = rel
+    out['differentiation_nxy2048_nvis2e6'] = res
+
+    # batched forward: 256 walkers x datasets (512, 1e4), (1024, 2e5), (2048, 1e6); Nr=300
+    rng = np.random.default_rng(3)
+    Rmin, dR, dxy = 1e-4 * A, 2.0 * A / 300, 0.03 * A
+    base = syn.ProfileWorkload(intensity=syn.ring_envelope_profile(300, Rmin, dR), Rmin=Rmin, dR=dR, nxy=0,
+                               dxy=dxy, inc=40 * syn.DEG, PA=120 * syn.DEG, dRA=0.02 * A, dDec=0.02 * A, u=None, v=None)
+    sets = []
+    for nxy, nvis in ((512, 10_000), (1024, 200_000), (2048, 1_000_000)):
+        base.nxy = nxy
+        base.u, base.v = syn.log_uniform_uv(rng, nvis, 3e4, 3e6)
+        re, im, w = syn.noisy_observation(rng, model(base))
+        sets.append((ta.Dataset(base.u, base.v, re, im, w, device=dev, dtype=f64), nxy, dxy))
+    I256, G256 = syn.walker_ensemble(rng, 256, 300, Rmin, dR, base.inc, base.PA, base.dRA, base.dDec)
+    plan = ta.BatchedChi2Profile(sets, 300, Rmin, dR, max_walkers=256, check_central_pixel=False)
+    It, Gt = torch.from_numpy(I256).to(dev), torch.from_numpy(G256).to(dev)
+    ms = timeit(lambda: plan(It, Gt), 10)
+    chunk, kw, _ = plan.info(256)
+    per = plan(It, Gt, return_sets=True)[1][0].cpu().numpy()
+    ref = [ds.chi2_profile(It[0], Rmin, dR, nxy, dx, check_central_pixel=False, inc=float(G256[0, 0]),
+                           PA=float(G256[0, 1]), dRA=float(G256[0, 2]), dDec=float(G256[0, 3])).item()
+           for ds, nxy, dx in sets]
+    out['batched_256_walkers_3_datasets_fwd'] = {
+        'workload': '256 walkers (Nr=300) x datasets (nxy, Nvis) = (512, 1e4), (1024, 2e5), (2048, 1e6), FP64, '
+                    'per-walker chi2 summed over the datasets',
+        'walker_evals_per_s': round(256 * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4), 'chunk': chunk,
+        'walkers_per_sample': kw,
+        'chi2_rel_diff_vs_plain': float(np.max(np.abs(per - ref) / np.abs(ref)))}
+    plan.close()
+    del sets
+
+    # batched forward+gradient: 64 sets x 3 datasets at nxy=4096 (Nvis 5e5, 1.5e6, 3e6), Nr=500
+    rng = np.random.default_rng(4)
+    Rmin, dR, dxy, nxy = 1e-4 * A, 0.004 * A, 0.01 * A, 4096
+    base = syn.ProfileWorkload(intensity=syn.ring_envelope_profile(500, Rmin, dR), Rmin=Rmin, dR=dR, nxy=nxy,
+                               dxy=dxy, inc=40 * syn.DEG, PA=120 * syn.DEG, dRA=0.02 * A, dDec=0.02 * A, u=None, v=None)
+    raw = []
+    for nvis in (500_000, 1_500_000, 3_000_000):
+        base.u, base.v = syn.log_uniform_uv(rng, nvis, 5e3, 1e7)
+        raw.append((base.u, base.v) + tuple(syn.noisy_observation(rng, model(base))))
+    I64, G64 = syn.walker_ensemble(rng, 64, 500, Rmin, dR, base.inc, base.PA, base.dRA, base.dDec)
+    res = {'workload': '64 parameter sets (Nr=500) x 3 datasets at nxy=4096 (Nvis 5e5, 1.5e6, 3e6; 5e6 total), '
+                       'chi2 + gradient per set'}
+    ref = None
+    for name, dt in (('f64', torch.float64), ('f32', torch.float32)):
+        sets = [(ta.Dataset(*r, device=dev, dtype=dt), nxy, dxy) for r in raw]
+        plan = ta.BatchedChi2Profile(sets, 500, Rmin, dR, max_walkers=64, check_central_pixel=False)
+        It, Gt = torch.from_numpy(I64).to(dev, dt), torch.from_numpy(G64).to(dev, dt)
+        ms = timeit(lambda: plan.grad(It, Gt), 5)
+        c, _, gI, gg = plan.grad(It, Gt)
+        chunk, _, _ = plan.info(64, grad=True)
+        if ref is None:   # FP64 plain path, set 0
+            sets64 = sets
+            r = [ds.chi2_profile_grad(It[0], Rmin, dR, nxy, dxy, check_central_pixel=False, inc=float(G64[0, 0]),
+                                      PA=float(G64[0, 1]), dRA=float(G64[0, 2]), dDec=float(G64[0, 3]))
+                 for ds, _, _ in sets64]
+            ref = (sum(x[0].item() for x in r), sum(x[1] for x in r), sum(x[2] for x in r))
+        res[name] = {'fwd_grad_set_evals_per_s': round(64 * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4),
+                     'chunk': chunk,
+                     'chi2_rel_err_vs_f64_plain': abs(float(c[0]) - ref[0]) / abs(ref[0]),
+                     'grad_I_rel_err_vs_f64_plain': float(torch.linalg.norm(gI[0] - ref[1]) / torch.linalg.norm(ref[1])),
+                     'grad_geom_max_rel_err_vs_f64_plain': float(torch.max(torch.abs(gg[0] - ref[2]) / torch.abs(ref[2])))}
+        plan.close()
+        del sets, plan
+    out['batched_64_sets_3_datasets_nxy4096_grad'] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -150,6 +270,7 @@ def main():
                     help='parameter-set counts of the batched form (comma list; empty: skip)')
     ap.add_argument('--batch-grad', type=str, default='1,8,64',
                     help='parameter-set counts of the batched forward+gradient')
+    ap.add_argument('--extras', type=int, default=1, help='also time the other configurations (0: skip)')
     ap.add_argument('--copy-calibration-mb', type=int, default=0,
                     help='first run 5 device-to-device copies of this many MiB (HBM byte-counter calibration)')
     args = ap.parse_args()
@@ -277,6 +398,23 @@ def main():
     if args.batch:
         batched = batched_rates(ta, ds, wl, I, chi2_dev, args.batch, args.batch_grad, args.steps, stream)
 
+    # plain forward in FP32 on the same workload
+    ds32 = ta.Dataset(wl.u, wl.v, wl.re, wl.im, wl.w, device=dev, dtype=torch.float32)
+    I32 = I.to(torch.float32)
+
+    def step32():
+        ds32.chi2_profile(I32, wl.Rmin, wl.dR, wl.nxy, wl.dxy, check_central_pixel=False, out=out, **kw)
+
+    for _ in range(args.warmup):
+        step32()
+    ms32, _, _, _ = run_timed(step32, args.steps)
+    c32 = float(out.item())
+    plain_f32 = {'evals_per_s': round(1000.0 / ms32, 2), 'ms_per_step': round(ms32, 5),
+                 'chi2_rel_err_vs_f64': abs(c32 - chi2_dev) / abs(chi2_dev)}
+    del ds32
+
+    extras = extra_configs(ta, g, stream, args.steps) if args.extras else None
+
     # host-pointer (drop-in) form, PCIe uploads included: a few calls
     nh = 5
     t = time.perf_counter()
@@ -326,7 +464,9 @@ def main():
                          'alg_bytes_per_eval': b_alg_grad,
                          'frac': round(b_alg_grad * evals_grad / (HBM_PEAK_GBS * 1e9), 4),
                          'stages': stages_grad},
+        'plain_f32': plain_f32,
         'batched': batched,
+        'other_configs': extras,
         'host_api_evals_per_s_incl_pcie': round(host_rate, 2),
         'cpu_baseline': cpu,
     }

@@ -539,9 +539,10 @@ void launch_chi2(Workspace& w, hipStream_t s, SampleArgs<T> a, const T* re, cons
     a.w = wt;
     a.partials = w.part.get<double>(nb ? nb : 1);
     t_timer.mark(ST_SAMPLE, s);
+    const unsigned ng = (nb + SAMPLE_PPT - 1) / SAMPLE_PPT;
     for (int r = t_timer.reps(ST_SAMPLE); r > 0; --r) {
-        if (nb && a.H) hipLaunchKernelGGL((k_sample<T, MODE_CHI2, true>), dim3(nb), dim3(256), 0, s, a);
-        else if (nb) hipLaunchKernelGGL((k_sample<T, MODE_CHI2, false>), dim3(nb), dim3(256), 0, s, a);
+        if (nb && a.H) hipLaunchKernelGGL((k_sample_chi2_n<T, true, SAMPLE_PPT>), dim3(ng), dim3(256), 0, s, a, int(nb));
+        else if (nb) hipLaunchKernelGGL((k_sample_chi2_n<T, false, SAMPLE_PPT>), dim3(ng), dim3(256), 0, s, a, int(nb));
     }
     t_timer.mark(ST_REDUCE, s);
     hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, a.partials, int(nb), chi2_d, chi2_t);

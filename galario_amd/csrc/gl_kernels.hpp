@@ -601,6 +601,47 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs<T> a) {
     sample_body<T, MODE, SYM>(a, blockIdx.x);
 }
 
+// chi2 with PPT blocks of 256 points per workgroup (more loads in flight per
+// wave); block q's partial is the same fixed-order sum as in k_sample.
+#ifndef GL_SAMPLE_PPT
+#define GL_SAMPLE_PPT 2
+#endif
+constexpr int SAMPLE_PPT = GL_SAMPLE_PPT;
+
+template <typename T, bool SYM, int PPT>
+__global__ __launch_bounds__(256) void k_sample_chi2_n(SampleArgs<T> a, int nblocks) {
+    __shared__ double red[PPT][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double acc[PPT];
+    bool oob_any = false;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        acc[q] = 0.0;
+        const int i = (blockIdx.x * PPT + q) * 256 + tid;
+        if (i < a.nd) {
+            bool oob = false;
+            const cpx<T> V = SYM ? sample_point_sym(a, a.u[i], a.v[i], oob) : sample_point(a, a.u[i], a.v[i], oob);
+            oob_any |= oob;
+            const T sw = sqrt(a.w[i]);
+            const T cr = (V.re - a.re[i]) * sw;
+            const T ci = (V.im - a.im[i]) * sw;
+            if constexpr (std::is_same<T, double>::value) acc[q] = cr * cr + ci * ci;
+            else acc[q] = double(cr) * cr + double(ci) * ci;
+        }
+    }
+    if (oob_any && a.flag) atomicOr(a.flag, 1u);
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const double x = wave_sum(acc[q]);
+        if (lane == 0) red[q][wv] = x;
+    }
+    __syncthreads();
+    if (tid < PPT) {
+        const int b = blockIdx.x * PPT + tid;
+        if (b < nblocks) a.partials[b] = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // expert stages
 // ---------------------------------------------------------------------------

@@ -138,3 +138,17 @@ def ring_image_workload(model, nvis: int = 100_000, nxy: int = 1024, seed: int =
     wl.u, wl.v = log_uniform_uv(rng, nvis, 3e4, 3e6)
     wl.re, wl.im, wl.w = noisy_observation(rng, model(wl))
     return wl
+
+
+def walker_ensemble(rng, nwalkers: int, nr: int, Rmin: float, dR: float, inc: float, PA: float, dRA: float,
+                    dDec: float, jitter: float = 0.1):
+    """Parameter sets around ring_envelope_profile(nr, Rmin, dR) and (inc, PA, dRA, dDec):
+    per-bin lognormal jitter of width `jitter` on the profile; inc + 0.05 N, PA + 0.2 N,
+    dRA/dDec + 0.005" N.  Returns (intensity [W, nr], geom [W, 4]).
+    Draws: nwalkers*nr normals (profile), then nwalkers*4 normals (geometry)."""
+    base = ring_envelope_profile(nr, Rmin, dR)
+    I = base[None, :] * np.exp(jitter * rng.standard_normal((nwalkers, nr)))
+    z = rng.standard_normal((nwalkers, 4))
+    geom = np.stack([inc + 0.05 * z[:, 0], PA + 0.2 * z[:, 1], dRA + 0.005 * ARCSEC * z[:, 2],
+                     dDec + 0.005 * ARCSEC * z[:, 3]], axis=1)
+    return I, geom
