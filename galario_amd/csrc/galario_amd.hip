@@ -949,6 +949,9 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
                                          " is not supported by the gradient (power of two from 128 to " +
                                          std::to_string(1 << max_fast_log2<T>()) + ")");
         if (!grad_I || !grad_geom || !chi2_d) invalid("chi2 gradient", "null output pointer");
+        if (nr > ADJROW_LDS_NR)
+            invalid("chi2 gradient", "nr = " + std::to_string(nr) + " profile bins (the gradient supports up to " +
+                                         std::to_string(ADJROW_LDS_NR) + ")");
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
@@ -1384,6 +1387,9 @@ int batch_run(void* h, void* stream, int W, const T* I, const T* geom, double* c
             invalid("batch", "nwalkers = " + std::to_string(W) + " (plan holds 1 to " + std::to_string(P.max_w) + ")");
         if (!I || !geom) invalid("batch", "null parameter array");
         if (grad && (!gI || !ggeom)) invalid("batch", "null gradient output");
+        if (grad && P.nr > ADJROW_LDS_NR)
+            invalid("batch", "nr = " + std::to_string(P.nr) + " profile bins (the gradient supports up to " +
+                                 std::to_string(ADJROW_LDS_NR) + ")");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (P.has_last && P.last != s) {
             HC(hipEventRecord(P.ev, P.last));

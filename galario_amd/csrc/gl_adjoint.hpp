@@ -206,10 +206,14 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
             const double x = hot[j];
             if (x != 0.0) atomic_add_f64(rep + j, x);
         }
+        // window cell j = c * wm + mm; c from a float reciprocal (exact after one correction, j < 2^24)
+        const float inv_wm = 1.0f / float(wm);
         for (int j = tid; j < nwin; j += 256) {
             const double x = win[j];
             if (x != 0.0) {
-                const int c = j / wm;
+                int c = int(float(j) * inv_wm);
+                c += (c + 1) * wm <= j;
+                c -= c * wm > j;
                 atomic_add_f64(g.Hh + size_t(umin + c) * a.ldH + (mmin + (j - c * wm)), x);
             }
         }
@@ -350,7 +354,8 @@ struct AdjRowArgs {
     double* ginc;        // 1, zeroed by the forward row pass
 };
 
-// dchi2/d(pixel value) = gz: accumulate into dI and return the inc term.
+// dchi2/d(pixel value) = gz: accumulate into dI (workgroup-local copy in LDS)
+// and return the inc term.
 template <typename T>
 __device__ __forceinline__ double sweep_pixel_adjoint(int xi, int yi, double gz, const SweepPrm<T>& p, double tan_inc,
                                                       double* gI) {
@@ -374,8 +379,8 @@ __device__ __forceinline__ double sweep_pixel_adjoint(int xi, int yi, double gz,
     const int iR = int(fi);
     const double alpha = (double(r) - double(iR) * double(p.dR) - double(p.Rmin)) / double(p.dR);
     const double sr = double(p.sr_to_px);
-    atomic_add_f64(gI + iR, gz * sr * (1.0 - alpha));
-    atomic_add_f64(gI + iR + 1, gz * sr * alpha);
+    lds_add_f64(gI + iR, gz * sr * (1.0 - alpha));
+    lds_add_f64(gI + iR + 1, gz * sr * alpha);
     if (r == T(0)) return 0.0;
     const double dvdr = sr * (double(p.I[iR + 1]) - double(p.I[iR])) / double(p.dR);
     return gz * dvdr * xs2 * tan_inc / double(r);
@@ -389,11 +394,13 @@ __device__ __noinline__ void central_adjoint(double gz, const SweepPrm<T>& p, do
     const double sc = gz * double(p.sr_to_px) / area;
     const double RiIN = Rmin + iIN * dR;
     const double delta = h - RiIN, beta = delta / dR;
-    atomic_add_f64(gI + 0, sc * dR * Rmin);
-    for (int i = 1; i < iIN; ++i) atomic_add_f64(gI + i, sc * 2.0 * dR * (Rmin + i * dR));
-    atomic_add_f64(gI + iIN, sc * (dR * RiIN + delta * (RiIN + h * (1.0 - beta))));
-    atomic_add_f64(gI + iIN + 1, sc * delta * h * beta);
+    lds_add_f64(gI + 0, sc * dR * Rmin);
+    for (int i = 1; i < iIN; ++i) lds_add_f64(gI + i, sc * 2.0 * dR * (Rmin + i * dR));
+    lds_add_f64(gI + iIN, sc * (dR * RiIN + delta * (RiIN + h * (1.0 - beta))));
+    lds_add_f64(gI + iIN + 1, sc * delta * h * beta);
 }
+
+constexpr int ADJROW_LDS_NR = 2048;   // profile bins accumulated in LDS by the adjoint row pass
 
 template <typename T, int LOG2N>
 __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk) {
@@ -402,7 +409,9 @@ __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk)
     constexpr int N = S::N, NT = S::NT, E = S::E;
     __shared__ T lds[S::LDS_ELEMS];
     __shared__ double red[16];
+    __shared__ double gIl[ADJROW_LDS_NR];    // this workgroup's share of dchi2/dI (nr <= ADJROW_LDS_NR)
     const int tid = threadIdx.x;
+    for (int k = tid; k < a.sw.nr; k += NT) gIl[k] = 0.0;   // ordered before use by the FFT's barriers
     const int ma = 2 * blk, mb = ma + 1;
     const bool hb = mb < a.nrows;
     const T* ra = a.Th + size_t(ma) * a.ldTh;
@@ -420,13 +429,17 @@ __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk)
         if (r * NT <= a.Rs && j <= a.Rs) {
             const double cj = mult(j, N);
             const double gza = cj * double(v[r].re), gzb = cj * double(v[r].im);
-            if (ma == 0 && j == 0) central_adjoint(gza, a.sw, a.gI);
-            else ginc += sweep_pixel_adjoint(j, ma, gza, a.sw, a.tan_inc, a.gI);
-            if (hb) ginc += sweep_pixel_adjoint(j, mb, gzb, a.sw, a.tan_inc, a.gI);
+            if (ma == 0 && j == 0) central_adjoint(gza, a.sw, gIl);
+            else ginc += sweep_pixel_adjoint(j, ma, gza, a.sw, a.tan_inc, gIl);
+            if (hb) ginc += sweep_pixel_adjoint(j, mb, gzb, a.sw, a.tan_inc, gIl);
         }
     }
-    const double b = block_sum_any(ginc, red);
+    const double b = block_sum_any(ginc, red);    // (its barrier also orders the LDS sums before the flush)
     if (tid == 0 && b != 0.0) atomic_add_f64(a.ginc, b);
+    for (int k = tid; k < a.sw.nr; k += NT) {      // consecutive bins: coalesced atomics
+        const double x = gIl[k];
+        if (x != 0.0) atomic_add_f64(a.gI + k, x);
+    }
 }
 
 template <typename T, int LOG2N>

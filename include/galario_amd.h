@@ -190,7 +190,7 @@ int galario_sample_profile_dev_f32(void *stream, int nr, const float *intensity,
 /* chi2 of a profile and its gradient (no counterpart in galario): *chi2_d,
  * grad_I[0..nr) = dchi2/dintensity, grad_geom[0..3] = dchi2/d(inc, PA, dRA,
  * dDec), all device doubles, enqueued on `stream`.  Requires nxy a power of
- * two in [128, 8192] (double) or [128, 16384] (float).  dchi2/dintensity is
+ * two in [128, 8192] (double) or [128, 16384] (float) and nr <= 2048.  dchi2/dintensity is
  * accumulated with atomics (summation order, hence the last bits, may vary
  * between calls). */
 int galario_chi2_profile_grad_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
