@@ -985,17 +985,17 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         t_timer.mark(ST_SAMPLE, s);
         w.Hh_zero = false;
         if (nb) hipLaunchKernelGGL((k_sample_grad_sym<T>), dim3(nb), dim3(256), 0, s, sa, ga);
-        t_timer.mark(ST_REDUCE, s);
-        Dst4 dst{{chi2_d, grad_geom + 1, grad_geom + 2, grad_geom + 3}};
-        hipLaunchKernelGGL(k_sum_partials4, dim3(1), dim3(1024), 0, s, part, int(nb), dst);
         HC(hipGetLastError());
-        // adjoint passes
+        // adjoint passes; workgroup 0 of the adjoint column pass also sums the partials into chi2 and
+        // the PA, dRA, dDec gradients (the order of k_sum_partials)
         const int ldTh = (g.C + 1 + 7) & ~7;
         T* Th = w.Th.get<T>(size_t(g.nrows) * ldTh);
         const cpx<T>* tw = twiddles<T>(nxy);
-        AdjColArgs<T> aca{Hh, hot, g.ldH, nxy, g.Rs, tw, Th, ldTh};
+        const Dst4 dst{{chi2_d, grad_geom + 1, grad_geom + 2, grad_geom + 3}};
+        const int nfft = 8 * ((g.pairs_c + 7) / 8);
+        AdjColArgs<T> aca{Hh, hot, g.ldH, nxy, g.Rs, tw, Th, ldTh, part, int(nb), dst, nfft};
         t_timer.mark(ST_ADJCOL, s);
-        launch_adjcol_sym<T>(L, 8 * ((g.pairs_c + 7) / 8), s, aca);
+        launch_adjcol_sym<T>(L, nfft + 8, s, aca);   // + 8 workgroups for the partial sums
         HC(hipGetLastError());
         w.Hh_zero = true;
         AdjRowArgs<T> ara{ps.sw, Th, ldTh, nxy, g.Rs, g.nrows, tw, std::tan(double(inc)), grad_I, grad_geom};
@@ -1206,7 +1206,7 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, 
                     launch_sample_multi<T>(kw, dim3(P.nb[si], (nw + kw - 1) / kw), s, a, b, nw);
             }
             if (grad) {
-                AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0};
+                AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0, nullptr, 0, Dst4{}, ablocks};
                 aca.ldTh = int(P.sTh / size_t(g.nrows));
                 AdjRowArgs<T> ara{G.sw, nullptr, aca.ldTh, G.N, g.Rs, g.nrows, tw, 0.0, nullptr, nullptr};
 #define GL_CALL(LL)                                                                                              \

@@ -238,27 +238,10 @@ __global__ __launch_bounds__(256) void k_sample_grad_sym(SampleArgs<T> a, GradAr
     sample_grad_sym_body<T>(a, g, blockIdx.x);
 }
 
-// Deterministic sums of 4 streams of n partials: *dst[j] = sum_i p[j*n + i].
+// Destinations of the 4 partial streams (chi2, dPA, ddRA, ddDec).
 struct Dst4 {
     double* d[4];
 };
-
-__global__ __launch_bounds__(1024) void k_sum_partials4(const double* __restrict__ p, int n, Dst4 dst) {
-    __shared__ double red[16];
-    for (int j = 0; j < 4; ++j) {
-        double acc = 0.0;
-        for (int i = threadIdx.x; i < n; i += 1024) acc += p[size_t(j) * n + i];
-        acc = wave_sum(acc);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int w = 0; w < 16; ++w) s += red[w];
-            if (dst.d[j]) *dst.d[j] = s;
-        }
-        __syncthreads();
-    }
-}
 
 // ---------------------------------------------------------------------------
 // adjoint of the column pass: Hh (two columns per workgroup) -> Th[m][l],
@@ -273,7 +256,40 @@ struct AdjColArgs {
     const cpx<T>* tw;
     T* Th;               // row-major [Rs'+1][ldTh]
     int ldTh;
+    const double* part;  // optional: 4 streams of np partials (stride np), each summed into dst by one of
+    int np;              // 8 extra workgroups after the nfft FFT workgroups
+    Dst4 dst;
+    int nfft;
 };
+
+// Deterministic sum *dst = sum_i p[i] of n partials in the order of
+// k_sum_partials (1024 virtual threads, thread t summing p[t], p[t + 1024],
+// ...; the 16 virtual waves' sums added in order), so chi2 equals the forward
+// call's bit for bit; run by one workgroup of NT threads, NT a multiple of 64
+// dividing 1024.
+template <int NT>
+__device__ __forceinline__ void sum_partials_in_block(const double* __restrict__ p, int n, double* dst,
+                                                      double* red16) {
+    constexpr int V = 1024 / NT;              // virtual threads per thread
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    double acc[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        acc[q] = 0.0;
+        for (int i = t + q * NT; i < n; i += 1024) acc[q] += p[i];
+    }
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        const double x = wave_sum(acc[q]);
+        if (lane == 0) red16[wv + q * (NT / 64)] = x;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double s = 0.0;
+        for (int w = 0; w < 16; ++w) s += red16[w];
+        if (dst) *dst = s;
+    }
+}
 
 __device__ __forceinline__ double mult(int m, int N) { return (m == 0 || m == N / 2) ? 1.0 : 2.0; }
 __device__ __forceinline__ double ext_weight(int k, int N) { return (k == 0 || k == N / 2) ? 1.0 : 0.5; }
@@ -286,7 +302,14 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
     constexpr int C = N / 2 + 1;
     constexpr int NPAIR = (C + 1) / 2;
     __shared__ T lds[S::LDS_ELEMS];
-    const int pair = xcd_column(blk, nblk);
+    static_assert(NT >= 64 && 1024 % NT == 0, "partial sums replay 1024 virtual threads");
+    if (blk >= a.nfft) {                    // extra workgroups: the partial sums of the sample kernel
+        const int j = blk - a.nfft;
+        __shared__ double red16[16];
+        if (a.part && j < 4) sum_partials_in_block<NT>(a.part + size_t(j) * a.np, a.np, a.dst.d[j], red16);
+        return;
+    }
+    const int pair = xcd_column(blk, a.nfft);
     if (pair >= NPAIR) return;
     const int la = 2 * pair, lb = la + 1;
     const bool has_b = lb < C;

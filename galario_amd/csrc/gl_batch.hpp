@@ -204,7 +204,7 @@ struct BatchSets {
 
 // One workgroup per walker: chi2 of each dataset (the single-call
 // k_sum_partials order), their sum over datasets, and with nstream = 4 the
-// PA, dRA, dDec gradients (k_sum_partials4 order per dataset, summed over
+// PA, dRA, dDec gradients (the same order per dataset, summed over
 // datasets).
 __global__ __launch_bounds__(1024) void k_batch_reduce(const double* __restrict__ part, int ldPart, BatchSets sets,
                                                        int nstream, double* chi2_sets, double* chi2_tot,
