@@ -157,3 +157,17 @@ def test_batch_plan_argument_checks_before_any_gpu_work():
         _lib.check(status([ds(512, 0.01 * a)], max_w=0))
     with pytest.raises(ValueError, match="not a batch plan"):
         _lib.check(_lib.lib.galario_batch_info(ctypes.c_void_p(0), 1, 0, None, None, None))
+
+
+def test_gradient_argument_checks_before_any_gpu_work():
+    """The gradient entry point rejects unsupported sizes on the host."""
+    a = galario_amd.arcsec
+    f = _lib.lib.galario_chi2_profile_grad_dev_f64
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p] + [ctypes.c_double] * 3 + [ctypes.c_int] + \
+        [ctypes.c_double] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 3
+    p = ctypes.c_void_p(0x1000)
+    common = (0.01 * a, 1024, 0.5, 0.0, 0.0, 1 / (1024 * 0.01 * a), 0.3, 10, p, p, p, p, p, 0, p, p, p)
+    with pytest.raises(ValueError, match="up to 2048"):
+        _lib.check(f(None, 3000, p, 1e-4 * a, 0.0005 * a, *common))
+    with pytest.raises(ValueError, match="not supported by the gradient"):
+        _lib.check(f(None, 1000, p, 1e-4 * a, 0.0005 * a, 0.01 * a, 1000, *common[2:]))
