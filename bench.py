@@ -61,7 +61,7 @@ def stage_bytes(nxy, nvis, nr, rmax, grad=False, s=8):
 
 
 KERNEL_NAMES = {'row_pass': 'k_rowpass_sym<double, 12>', 'column_pass': 'k_colpass_sym<double, 12>',
-                'sample': 'k_sample<double, 1, true>'}
+                'sample': 'k_sample_chi2_n<double, true, 2>'}
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hbm_traffic_profile_nxy4096_nvis1e6_f64.txt')
 
 
