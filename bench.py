@@ -303,7 +303,10 @@ def main():
                                check_central_pixel=False, **wl.kwargs())
 
     wl = syn.ring_profile_workload(gpu_model, nvis=args.nvis, nxy=args.nxy, seed=1)
+    t_up = time.perf_counter()
     ds = ta.Dataset(wl.u, wl.v, wl.re, wl.im, wl.w, device=dev, dtype=torch.float64)   # uploaded once, sorted
+    torch.cuda.synchronize()
+    upload_ms = (time.perf_counter() - t_up) * 1e3
     I = torch.from_numpy(np.ascontiguousarray(wl.intensity)).to(dev)
     out = torch.empty(1, dtype=torch.float64, device=dev)
     kw = wl.kwargs()
@@ -468,6 +471,7 @@ def main():
         'batched': batched,
         'other_configs': extras,
         'host_api_evals_per_s_incl_pcie': round(host_rate, 2),
+        'dataset_upload_ms': round(upload_ms, 2),
         'cpu_baseline': cpu,
     }
     print(json.dumps(res), flush=True)
