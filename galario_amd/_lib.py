@@ -109,6 +109,10 @@ def _load():
                                       _vp, _vp, _c_int, _vp, _vp, _vp],
             'sample_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _c_int,
                                    _vp, _vp],
+            'sample_profile_multi_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, _c_int, _vp, _c_int, _vp, _vp,
+                                         _c_int, _vp, _vp],
+            'chi2_profile_multi_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, _c_int, _vp, _c_int, _vp, _vp,
+                                       _vp, _vp, _vp, _c_int, _vp],
         }
         for name, args in sig.items():
             f = getattr(lib, f'galario_{name}_{suf}')

@@ -933,6 +933,46 @@ int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, in
     });
 }
 
+// Several sources sharing the profile, inc and PA at their own offsets and
+// fluxes (src: nsrc x 3 = flux, dRA, dDec, device): visibilities (vis_d,
+// written through perm when given) or chi2 (chi2_d).
+template <typename T>
+int profile_multisrc_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T duv, T PA, int nsrc,
+                         const T* src, int nd, const T* u, const T* v, const T* re, const T* im, const T* wt,
+                         int check_central, const int* perm, void* vis_d, double* chi2_d) {
+    return guard([&] {
+        check_nd(nd);
+        if (nsrc < 0) invalid("multiple sources", "nsrc = " + std::to_string(nsrc) + " is negative");
+        if (nsrc > 0 && !src) invalid("multiple sources", "null source array");
+        ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
+        SampleSetup<T> ss = setup_sample<T>(PA, T(0), T(0));
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        Workspace& w = workspace(s, false);
+        ps.sw.I = I;
+        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, u, v);
+        sa.flag = w.flag_dev.get<unsigned>(1);
+        const unsigned nb = blocks_for(size_t(nd), 256);
+        if (chi2_d) {
+            sa.re = re;
+            sa.im = im;
+            sa.w = wt;
+            sa.partials = w.part.get<double>(nb ? nb : 1);
+            t_timer.mark(ST_SAMPLE, s);
+            if (nb) hipLaunchKernelGGL((k_sample_multisrc<T, MODE_CHI2>), dim3(nb), dim3(256), 0, s, sa, nsrc, src);
+            t_timer.mark(ST_REDUCE, s);
+            hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, sa.partials, int(nb), chi2_d,
+                               static_cast<T*>(nullptr));
+        } else {
+            sa.vis = static_cast<cpx<T>*>(vis_d);
+            sa.perm = perm;
+            t_timer.mark(ST_SAMPLE, s);
+            if (nb) hipLaunchKernelGGL((k_sample_multisrc<T, MODE_VIS>), dim3(nb), dim3(256), 0, s, sa, nsrc, src);
+        }
+        HC(hipGetLastError());
+        t_timer.end_call(s);
+    });
+}
+
 // chi2 and its gradient with respect to the profile (grad_I, nr doubles) and
 // to inc, PA, dRA, dDec (grad_geom[0..3]), all on the device.
 template <typename T>
@@ -1563,6 +1603,27 @@ GL_BOTH(GL_MAIN)
     }
 
 GL_BOTH(GL_GRAD)
+
+#define GL_MULTI(SUF, T)                                                                                          \
+    int galario_sample_profile_multi_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy,    \
+                                               T inc, T duv, T PA, int nsrc, const T* src, int nd, const T* u,    \
+                                               const T* v, int cc, const int* perm, void* vis_d) {                \
+        return profile_multisrc_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, duv, PA, nsrc, src, nd, u, v,      \
+                                       nullptr, nullptr, nullptr, cc, perm, vis_d, nullptr);                      \
+    }                                                                                                             \
+    int galario_chi2_profile_multi_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy,      \
+                                             T inc, T duv, T PA, int nsrc, const T* src, int nd, const T* u,      \
+                                             const T* v, const T* re, const T* im, const T* w, int cc,            \
+                                             double* chi2_d) {                                                    \
+        if (!chi2_d) {                                                                                            \
+            t_err = "Error in multiple sources: null chi2 pointer";                                               \
+            return GALARIO_ERR_INVALID;                                                                           \
+        }                                                                                                         \
+        return profile_multisrc_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, duv, PA, nsrc, src, nd, u, v, re,  \
+                                       im, w, cc, nullptr, nullptr, chi2_d);                                      \
+    }
+
+GL_BOTH(GL_MULTI)
 
 #define GL_BATCH(SUF, T)                                                                                          \
     int galario_batch_create_##SUF(int nsets, const galario_dataset_##SUF* sets, int nr, T Rmin, T dR,           \

@@ -601,6 +601,64 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs<T> a) {
     sample_body<T, MODE, SYM>(a, blockIdx.x);
 }
 
+// Several sources with the same profile, inclination and PA at different
+// offsets (galario's multi-source recipe: the visibilities of each source,
+// phase-shifted by its own offset, summed; docs/tech-specs.rst:45-57).  They
+// share the spectrum and the interpolated value V0 at each point, so
+// V = sum_k flux_k * apply_phase(V0, dRA_k, dDec_k), k in order.
+// src: nsrc x 3 (flux, dRA, dDec) in the caller's (unrotated) frame.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_sample_multisrc(SampleArgs<T> a, int nsrc, const T* __restrict__ src) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    bool oob = false;
+    if (i < a.nd) {
+        SampleArgs<T> b = a;
+        b.phase = 0;
+        const cpx<T> V0 = a.H ? sample_point_sym(b, a.u[i], a.v[i], oob) : sample_point(b, a.u[i], a.v[i], oob);
+        T ur = a.u[i], vr = a.v[i];
+        if (a.rotate) {
+            ur = a.u[i] * a.cosPA - a.v[i] * a.sinPA;
+            vr = a.u[i] * a.sinPA + a.v[i] * a.cosPA;
+        }
+        cpx<T> V{T(0), T(0)};
+        for (int k = 0; k < nsrc; ++k) {
+            const T f = src[3 * k], dRA = src[3 * k + 1], dDec = src[3 * k + 2];
+            T dRAr = dRA, dDecr = dDec;
+            if (a.rotate) {
+                dRAr = dRA * a.cosPA - dDec * a.sinPA;
+                dDecr = dRA * a.sinPA + dDec * a.cosPA;
+            }
+            cpx<T> Vk = V0;
+            if (!(dRAr == T(0) && dDecr == T(0))) {
+                const T ang = ur * T(double(dRAr) * (2. * double(T(M_PI)))) +
+                              vr * T(double(dDecr) * (2. * double(T(M_PI))));
+                T sn, cs;
+                fsincos(ang, &sn, &cs);
+                Vk = cpx<T>{V0.re * cs - V0.im * sn, V0.re * sn + V0.im * cs};
+            }
+            V.re += f * Vk.re;
+            V.im += f * Vk.im;
+        }
+        if (oob) V = cpx<T>{T(NAN), T(NAN)};
+        if constexpr (MODE == MODE_VIS) {
+            a.vis[a.perm ? a.perm[i] : i] = V;
+        } else {
+            const T sw = sqrt(a.w[i]);
+            const T cr = (V.re - a.re[i]) * sw;
+            const T ci = (V.im - a.im[i]) * sw;
+            if constexpr (std::is_same<T, double>::value) acc = cr * cr + ci * ci;
+            else acc = double(cr) * cr + double(ci) * ci;
+        }
+    }
+    if (oob && a.flag) atomicOr(a.flag, 1u);
+    if constexpr (MODE == MODE_CHI2) {
+        const double b = block_sum256(acc, red);
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = b;
+    }
+}
+
 // chi2 with PPT blocks of 256 points per workgroup (more loads in flight per
 // wave); block q's partial is the same fixed-order sum as in k_sample.
 #ifndef GL_SAMPLE_PPT

@@ -115,6 +115,45 @@ def stage_times():
     return dict(zip(STAGES, list(ms))), n.value
 
 
+def _sources(sources, dt, dev):
+    src = torch.as_tensor(sources, dtype=dt, device=dev).reshape(-1, 3).contiguous()
+    return src, src.shape[0]
+
+
+def sample_profile_multi(intensity, Rmin, dR, nxy, dxy, u, v, sources, PA=0., inc=0., check_central_pixel=True,
+                         out=None, perm=None):
+    """Visibilities of several sources with this profile, inc and PA: ``sources`` [K, 3] = (flux, dRA, dDec);
+    sum_k flux_k sampleProfile(..., dRA_k, dDec_k) from one spectrum (complex device tensor)."""
+    dt = intensity.dtype
+    dev = _check_tensors(dt, intensity, u, v)
+    src, k = _sources(sources, dt, dev)
+    nd = u.numel()
+    if out is None:
+        out = torch.empty(nd, dtype=torch.complex128 if dt == torch.float64 else torch.complex64, device=dev)
+    f = getattr(lib, f'galario_sample_profile_multi_dev_{_SUF[dt]}')
+    with torch.cuda.device(dev):
+        _check(f(_stream_ptr(dev), intensity.numel(), _p(intensity), Rmin, dR, dxy, int(nxy), inc, 1.0 / (dxy * nxy),
+                 PA, k, _p(src), nd, _p(u), _p(v), int(bool(check_central_pixel)),
+                 _p(perm) if perm is not None else None, _p(out)))
+    return out
+
+
+def chi2_profile_multi(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_im, vis_obs_w, sources, PA=0., inc=0.,
+                       check_central_pixel=True, out=None):
+    """chi2 of the multi-source model (see ``sample_profile_multi``); 1-element float64 device tensor."""
+    dt = intensity.dtype
+    dev = _check_tensors(dt, intensity, u, v, vis_obs_re, vis_obs_im, vis_obs_w)
+    src, k = _sources(sources, dt, dev)
+    if out is None:
+        out = torch.empty(1, dtype=torch.float64, device=dev)
+    f = getattr(lib, f'galario_chi2_profile_multi_dev_{_SUF[dt]}')
+    with torch.cuda.device(dev):
+        _check(f(_stream_ptr(dev), intensity.numel(), _p(intensity), Rmin, dR, dxy, int(nxy), inc, 1.0 / (dxy * nxy),
+                 PA, k, _p(src), u.numel(), _p(u), _p(v), _p(vis_obs_re), _p(vis_obs_im), _p(vis_obs_w),
+                 int(bool(check_central_pixel)), _p(out)))
+    return out
+
+
 def _spread16(x: np.ndarray) -> np.ndarray:
     x = x.astype(np.uint64) & 0xFFFF
     x = (x | (x << 8)) & 0x00FF00FF
@@ -175,6 +214,19 @@ class Dataset:
         """Visibilities in the caller's original point order."""
         return sample_profile(intensity, Rmin, dR, nxy, dxy, self.u, self.v, dRA=dRA, dDec=dDec, PA=PA, inc=inc,
                               check_central_pixel=check_central_pixel, out=out, perm=self.order)
+
+    def sample_profile_multi(self, intensity, Rmin, dR, nxy, dxy, sources, PA=0., inc=0., check_central_pixel=True,
+                             out=None):
+        """Multi-source visibilities in the caller's original point order."""
+        return sample_profile_multi(intensity, Rmin, dR, nxy, dxy, self.u, self.v, sources, PA=PA, inc=inc,
+                                    check_central_pixel=check_central_pixel, out=out, perm=self.order)
+
+    def chi2_profile_multi(self, intensity, Rmin, dR, nxy, dxy, sources, PA=0., inc=0., check_central_pixel=True,
+                           out=None):
+        if not self.has_obs:
+            raise ValueError('this Dataset has no observed visibilities')
+        return chi2_profile_multi(intensity, Rmin, dR, nxy, dxy, self.u, self.v, self.re, self.im, self.w, sources,
+                                  PA=PA, inc=inc, check_central_pixel=check_central_pixel, out=out)
 
 
 def chi2_profile_grad(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_im, vis_obs_w, dRA=0., dDec=0., PA=0.,

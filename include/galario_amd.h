@@ -204,6 +204,32 @@ int galario_chi2_profile_grad_dev_f32(void *stream, int nr, const float *intensi
                                       const float *vis_obs_re, const float *vis_obs_im, const float *weights,
                                       int check_central, double *chi2_d, double *grad_I, double *grad_geom);
 
+/* Several sources with the same profile, inclination and PA at their own
+ * offsets (galario's multi-source recipe, docs/tech-specs.rst:45-57: sum the
+ * visibilities of each source phase-shifted to its position; no counterpart
+ * function in galario).  src: nsrc x 3 device array (flux, dRA, dDec); the
+ * result is sum_k flux_k * sampleProfile(..., dRA_k, dDec_k), computed from
+ * one spectrum and one interpolation per point (sources in order).  nsrc = 1,
+ * flux 1 reproduces the single-source call bit for bit. */
+int galario_sample_profile_multi_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
+                                         double dxy, int nxy, double inc, double duv, double PA, int nsrc,
+                                         const double *src, int nd, const double *u, const double *v,
+                                         int check_central, const int *perm, void *vis_d);
+int galario_sample_profile_multi_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
+                                         float dxy, int nxy, float inc, float duv, float PA, int nsrc,
+                                         const float *src, int nd, const float *u, const float *v,
+                                         int check_central, const int *perm, void *vis_d);
+int galario_chi2_profile_multi_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
+                                       double dxy, int nxy, double inc, double duv, double PA, int nsrc,
+                                       const double *src, int nd, const double *u, const double *v,
+                                       const double *vis_obs_re, const double *vis_obs_im, const double *weights,
+                                       int check_central, double *chi2_d);
+int galario_chi2_profile_multi_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
+                                       float dxy, int nxy, float inc, float duv, float PA, int nsrc,
+                                       const float *src, int nd, const float *u, const float *v,
+                                       const float *vis_obs_re, const float *vis_obs_im, const float *weights,
+                                       int check_central, double *chi2_d);
+
 /* ---- batched form (no counterpart in galario, whose callers loop over
  *      chi2Profile per walker and dataset, docs/quickstart.rst:147-174) ---- */
 
