@@ -428,9 +428,9 @@ constexpr int ADJROW_LDS_NR = 2048;   // profile bins accumulated in LDS by the 
 template <typename T, int LOG2N>
 __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk) {
     using S = FftShape<LOG2N>;
-    using F = BlockFft<T, LOG2N, false>;
+    using F = BlockFft<T, LOG2N, false, false>;     // few workgroups: the lower-latency exchange
     constexpr int N = S::N, NT = S::NT, E = S::E;
-    __shared__ T lds[S::LDS_ELEMS];
+    __shared__ T lds[F::LDS_REALS];
     __shared__ double red[16];
     __shared__ double gIl[ADJROW_LDS_NR];    // this workgroup's share of dchi2/dI (nr <= ADJROW_LDS_NR)
     const int tid = threadIdx.x;

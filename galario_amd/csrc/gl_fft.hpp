@@ -162,10 +162,15 @@ struct FftShape {
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 
-template <typename T, int LOG2N, bool INV>
+// SPLIT: exchange real parts then imaginary parts through one buffer of
+// LDS_ELEMS reals (4 barriers per exchange, half the LDS: more workgroups per
+// CU); otherwise both parts at once through 2 * LDS_ELEMS reals (2 barriers:
+// lower latency, for kernels with few workgroups).
+template <typename T, int LOG2N, bool INV, bool SPLIT = true>
 struct BlockFft {
     using S = FftShape<LOG2N>;
     static constexpr int N = S::N, E = S::E, NT = S::NT;
+    static constexpr int LDS_REALS = SPLIT ? S::LDS_ELEMS : 2 * S::LDS_ELEMS;   // LDS the caller provides
     static constexpr int TWC = 8;           // twiddle bases per pass and thread
 
     static __device__ __forceinline__ cpx<T> twid(const cpx<T>* __restrict__ tw, int e) {
@@ -292,14 +297,23 @@ struct BlockFft {
         if constexpr (P < S::NPASS) {
             cpx<T> tc[TWC];
             preload<P>(tc, tid, tw);          // issued before the exchange, used after it
-            store_lds<P - 1, 0>(lds, v, tid);
-            __syncthreads();
-            load_lds<P, 0>(v, lds, tid);
-            __syncthreads();
-            store_lds<P - 1, 1>(lds, v, tid);
-            __syncthreads();
-            load_lds<P, 1>(v, lds, tid);
-            __syncthreads();
+            if constexpr (SPLIT) {
+                store_lds<P - 1, 0>(lds, v, tid);
+                __syncthreads();
+                load_lds<P, 0>(v, lds, tid);
+                __syncthreads();
+                store_lds<P - 1, 1>(lds, v, tid);
+                __syncthreads();
+                load_lds<P, 1>(v, lds, tid);
+                __syncthreads();
+            } else {
+                store_lds<P - 1, 0>(lds, v, tid);
+                store_lds<P - 1, 1>(lds + S::LDS_ELEMS, v, tid);
+                __syncthreads();
+                load_lds<P, 0>(v, lds, tid);
+                load_lds<P, 1>(v, lds + S::LDS_ELEMS, tid);
+                __syncthreads();
+            }
             if constexpr (P == S::NPASS - 1) {
                 if (only0) {
                     last_output0<P>(v, tc);

@@ -308,10 +308,10 @@ struct SymRowArgs {
 template <typename T, int LOG2N>
 __device__ __forceinline__ void rowpass_sym_body(const SymRowArgs<T>& a, int blk) {
     using S = FftShape<LOG2N>;
-    using F = BlockFft<T, LOG2N, false>;
+    using F = BlockFft<T, LOG2N, false, false>;     // few workgroups: the lower-latency exchange
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
-    __shared__ T lds[S::LDS_ELEMS];
+    __shared__ T lds[F::LDS_REALS];
     const int tid = threadIdx.x;
     const int ma = 2 * blk, mb = ma + 1;
     const bool hb = mb < a.nrows;
@@ -384,9 +384,9 @@ __device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk
     __shared__ T lds[S::LDS_ELEMS];
     const int pair = xcd_column(blk, nblk);
     if (pair >= NPAIR) return;
-    const int la = 2 * pair, lb = la + 1;
+    const int la = 2 * pair;
     const int tid = threadIdx.x;
-    const T* ca = a.Ts + size_t(la) * a.ldT;     // columns la, lb of Ts: contiguous runs of rows m
+    const T* ca = a.Ts + size_t(la) * a.ldT;     // columns la, la + 1 of Ts: contiguous runs of rows m
     const T* cb = ca + a.ldT;
     cpx<T> v[E];
     F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
