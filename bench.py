@@ -275,8 +275,8 @@ def main():
                     help='first run 5 device-to-device copies of this many MiB (HBM byte-counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU form is a design "
-              f"(see DESIGN.md), not a measured path in this build.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): sharding walkers or uv points "
+              f"over several GPUs is designed but not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
