@@ -178,16 +178,21 @@ Workspace& workspace(hipStream_t s, bool use_own) {
     HC(hipGetDevice(&dev));
     Workspace*& w = t_ws[dev];
     if (!w) {
-        w = new Workspace();
-        w->device = dev;
-        HC(hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking));
-        HC(hipEventCreateWithFlags(&w->ev, hipEventDisableTiming));
-        HC(hipHostMalloc(reinterpret_cast<void**>(&w->h_scal), 64, hipHostMallocDefault));
-        unsigned* fh = w->flag_host.get<unsigned>(1);
-        unsigned* fd = w->flag_dev.get<unsigned>(1);
-        HC(hipMemsetAsync(fh, 0, sizeof(unsigned), w->own));
-        HC(hipMemsetAsync(fd, 0, sizeof(unsigned), w->own));
-        HC(hipStreamSynchronize(w->own));
+        // built aside and published only when complete: a failure leaves no half-made workspace
+        struct Del {
+            void operator()(Workspace* p) const { p->release(); delete p; }
+        };
+        std::unique_ptr<Workspace, Del> n(new Workspace());
+        n->device = dev;
+        HC(hipStreamCreateWithFlags(&n->own, hipStreamNonBlocking));
+        HC(hipEventCreateWithFlags(&n->ev, hipEventDisableTiming));
+        HC(hipHostMalloc(reinterpret_cast<void**>(&n->h_scal), 64, hipHostMallocDefault));
+        unsigned* fh = n->flag_host.get<unsigned>(1);
+        unsigned* fd = n->flag_dev.get<unsigned>(1);
+        HC(hipMemsetAsync(fh, 0, sizeof(unsigned), n->own));
+        HC(hipMemsetAsync(fd, 0, sizeof(unsigned), n->own));
+        HC(hipStreamSynchronize(n->own));
+        w = n.release();
     }
     hipStream_t use = use_own ? w->own : s;
     if (w->has_last && w->last != use) {
