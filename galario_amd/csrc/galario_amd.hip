@@ -404,7 +404,7 @@ SymGeom sym_geom(int L, int N, int rmax) {
     g.nrows = g.Rs + 1;
     g.pairs_r = (g.nrows + 1) / 2;
     g.pairs_c = (g.C + 1) / 2;
-    g.ldT = 2 * g.pairs_r;                            // column stride of Ts (rows m, even)
+    g.ldT = (g.C + 7) & ~7;                           // reals per row of Ts (2 pairs_r rows, ts_index)
     g.ldH = (N / 2 + 1 + 7) & ~7;
     return g;
 }
@@ -419,7 +419,7 @@ bool build_spectrum_sym(Workspace& w, hipStream_t s, const SweepPrm<T>& sw, int 
     const int L = fast_log2<T>(N);
     if (L < 0) return false;
     const SymGeom g = sym_geom(L, N, rmax);
-    T* Ts = w.Tm.get<T>(size_t(g.C + 1) * g.ldT);            // C+1 columns: the last pair reads one past C
+    T* Ts = w.Tm.get<T>(size_t(2 * g.pairs_r) * g.ldT);      // rows 0..nrows-1 and a padding row
     T* H = w.S.get<T>(size_t(g.C + 1) * g.ldH);
     const cpx<T>* tw = twiddles<T>(N);
     SymRowArgs<T> ra{sw, N, g.Rs, g.nrows, g.ldT, tw, Ts, zero, nzero, zero1};
@@ -1329,7 +1329,7 @@ int batch_create(int nsets, const BatchSet<T>* sets, int nr, T Rmin, T dR, int m
         size_t maxHh = 0, maxTh = 0;
         for (const BatchGroup<T>& G : P->groups) {
             const SymGeom& g = G.g;
-            P->sTs = std::max(P->sTs, size_t(g.C + 1) * g.ldT);
+            P->sTs = std::max(P->sTs, size_t(2 * g.pairs_r) * g.ldT);
             P->sH = std::max(P->sH, size_t(g.C + 1) * g.ldH);
             maxHh = std::max(maxHh, size_t(g.C + 1) * g.ldH);
             const size_t ldTh = size_t((g.C + 1 + 7) & ~7);

@@ -286,18 +286,24 @@ __global__ __launch_bounds__(256) void k_coldft_generic(ColArgs<T> a) {
 // frequencies: the row pass packs rows m_a, m_b as one complex FFT and reads
 // the two real results off the real and imaginary parts; the column pass does
 // the same for two columns and stores frequencies 0..N/2 only.
-//   Ts : real, column-major [C+1][ldT]: Ts[l*ldT + m] = row transform of image
-//        row m = |y| pixel index (m = 0..Rs') at u-frequency l; column C is
-//        zero (partner of column C-1 in the last column pair).
+//   Ts : real, row transform of image row m = |y| pixel index (m = 0..Rs') at
+//        u-frequency l (l < C), in 64-byte blocks of 2 rows x 4 frequencies
+//        (ts_index): a row pass workgroup (two rows) writes whole lines, and a
+//        column pair reads 16 bytes per row, two rows per line.
 //   H  : real, column-major [C][ldH]: H[l*ldH + m] = S at u-column l and
 //        |v-frequency| m; S at shifted row r is H[.. + |r - N/2|].
 // ---------------------------------------------------------------------------
+// Offset of Ts element (row m, frequency l); ldT (>= C, a multiple of 8) reals per row.
+__device__ __forceinline__ size_t ts_index(int m, int l, int ldT) {
+    return (size_t(m >> 1) * size_t(ldT) + size_t(l >> 2) * 4) * 2 + size_t(((m & 1) << 2) + (l & 3));
+}
+
 template <typename T>
 struct SymRowArgs {
     SweepPrm<T> sw;
     int N, Rs;            // symmetric support half-width Rs' (columns [0,Rs') U [N-Rs',N))
     int nrows;            // Rs' + 1 rows m = 0..Rs'
-    int ldT;              // column stride of Ts (>= 2 * number of row pairs)
+    int ldT;              // reals per row of Ts (>= C, a multiple of 8; see ts_index)
     const cpx<T>* tw;
     T* Ts;
     double* zero;         // optional: workgroup 0 clears zero[0..nzero) and *zero1 (gradient accumulators)
@@ -327,15 +333,16 @@ __device__ __forceinline__ void rowpass_sym_body(const SymRowArgs<T>& a, int blk
         const T zb = hb ? sweep_pixel(xi, mb, a.sw) : T(0);
         return cpx<T>{za, zb};
     });
-    // rows ma, mb are adjacent in each column of Ts: one 16-byte store per frequency
-    cpx<T>* out = reinterpret_cast<cpx<T>*>(a.Ts + ma);
-    const size_t ld2 = size_t(a.ldT) / 2;
+    // rows ma, mb of Ts (mb = nrows is a padding row): four consecutive lanes fill one line
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         const int k = tid + r * NT;
-        if (r * NT < C && k < C) out[size_t(k) * ld2] = v[r];
+        if (r * NT < C && k < C) {
+            T* o = a.Ts + ts_index(ma, k, a.ldT);
+            o[0] = v[r].re;
+            o[4] = v[r].im;
+        }
     }
-    if (tid == 0) out[size_t(C) * ld2] = cpx<T>{T(0), T(0)};   // padding column C
 }
 
 template <typename T, int LOG2N>
@@ -361,14 +368,13 @@ __device__ __forceinline__ void colpass_nyquist(const SymColArgs<T>& a, int blk)
     constexpr int N = 1 << LOG2N, NT = FftShape<LOG2N>::NT;
     constexpr int NW = NT / 64 > 0 ? NT / 64 : 1;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const T* col = a.Ts + size_t(N / 2) * a.ldT;
     for (int q = wv; q < 3; q += NW) {
         const int k = blk + q * (N / 4);
         if (k > N / 2) break;
         double acc = 0.0;
         for (int m = lane; m <= a.Rs; m += 64) {
             const double wm = (m == 0 || m == a.Rs) ? 1.0 : 2.0;
-            acc += wm * double(col[m]) * double(a.tw[(m * k) & (N - 1)].re);
+            acc += wm * double(a.Ts[ts_index(m, N / 2, a.ldT)]) * double(a.tw[(m * k) & (N - 1)].re);
         }
         acc = wave_sum(acc);
         if (lane == 0) a.H[size_t(N / 2) * a.ldH + k] = T(acc);
@@ -386,12 +392,11 @@ __device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk
     if (pair >= NPAIR) return;
     const int la = 2 * pair;
     const int tid = threadIdx.x;
-    const T* ca = a.Ts + size_t(la) * a.ldT;     // columns la, la + 1 of Ts: contiguous runs of rows m
-    const T* cb = ca + a.ldT;
+    // columns la, la + 1 of Ts: one aligned 16-byte pair per row m (la even)
     cpx<T> v[E];
     F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
         const int m = n < N / 2 ? n : N - n;
-        return cpx<T>{ca[m], cb[m]};
+        return *reinterpret_cast<const cpx<T>*>(a.Ts + ts_index(m, la, a.ldT));
     });
     T* ha = a.H + size_t(la) * a.ldH;
     T* hb = ha + a.ldH;
