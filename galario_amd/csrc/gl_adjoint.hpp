@@ -101,8 +101,8 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
         cu1 = p.fu1;
         cm0 = abs(p.fl_v - half);
         cm1 = abs(p.fv1 - half);
-        const T y0 = a.H[size_t(cu0) * a.ldH + cm0], y1 = a.H[size_t(cu0) * a.ldH + cm1];
-        const T y3 = a.H[size_t(cu1) * a.ldH + cm0], y2 = a.H[size_t(cu1) * a.ldH + cm1];
+        const T y0 = a.H[h_index(cu0, cm0, a.ldH)], y1 = a.H[h_index(cu0, cm1, a.ldH)];
+        const T y3 = a.H[h_index(cu1, cm0, a.ldH)], y2 = a.H[h_index(cu1, cm1, a.ldH)];
         const T z = (t * (q * ((y0 + y2) - (y1 + y3))) + (t * (y1 - y0) + q * (y3 - y0))) + y0;
         const T a0 = fabs(y0), a1 = fabs(y1), a2 = fabs(y2), a3 = fabs(y3);
         T amp = a0;

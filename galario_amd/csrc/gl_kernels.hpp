@@ -290,9 +290,17 @@ __global__ __launch_bounds__(256) void k_coldft_generic(ColArgs<T> a) {
 //        u-frequency l (l < C), in 64-byte blocks of 2 rows x 4 frequencies
 //        (ts_index): a row pass workgroup (two rows) writes whole lines, and a
 //        column pair reads 16 bytes per row, two rows per line.
-//   H  : real, column-major [C][ldH]: H[l*ldH + m] = S at u-column l and
-//        |v-frequency| m; S at shifted row r is H[.. + |r - N/2|].
+//   H  : real, S at u-column l and |v-frequency| m (S at shifted row r is
+//        m = |r - N/2|), columns interleaved in pairs (h_index): column pair
+//        {2p, 2p+1} is a run of ldH (m, 2p), (m, 2p+1) pairs, so the column pass
+//        stores one 16-byte pair per row and a sample's four corners (columns
+//        c, c+1, rows m0, m1) share one 32-byte run when c is even.
 // ---------------------------------------------------------------------------
+// Offset of H element (column l, row m); ldH (>= N/2 + 1) rows per column.
+__device__ __forceinline__ size_t h_index(int l, int m, int ldH) {
+    return (size_t(l >> 1) * size_t(ldH) + size_t(m)) * 2 + size_t(l & 1);
+}
+
 // Offset of Ts element (row m, frequency l); ldT (>= C, a multiple of 8) reals per row.
 __device__ __forceinline__ size_t ts_index(int m, int l, int ldT) {
     return (size_t(m >> 1) * size_t(ldT) + size_t(l >> 2) * 4) * 2 + size_t(((m & 1) << 2) + (l & 3));
@@ -377,7 +385,7 @@ __device__ __forceinline__ void colpass_nyquist(const SymColArgs<T>& a, int blk)
             acc += wm * double(a.Ts[ts_index(m, N / 2, a.ldT)]) * double(a.tw[(m * k) & (N - 1)].re);
         }
         acc = wave_sum(acc);
-        if (lane == 0) a.H[size_t(N / 2) * a.ldH + k] = T(acc);
+        if (lane == 0) a.H[h_index(N / 2, k, a.ldH)] = T(acc);
     }
 }
 
@@ -398,15 +406,11 @@ __device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk
         const int m = n < N / 2 ? n : N - n;
         return *reinterpret_cast<const cpx<T>*>(a.Ts + ts_index(m, la, a.ldT));
     });
-    T* ha = a.H + size_t(la) * a.ldH;
-    T* hb = ha + a.ldH;
+    cpx<T>* hab = reinterpret_cast<cpx<T>*>(a.H + h_index(la, 0, a.ldH));   // columns la, la + 1
 #pragma unroll
     for (int r = 0; r <= E / 2; ++r) {
         const int k = tid + r * NT;
-        if (k <= N / 2) {
-            ha[k] = v[r].re;
-            hb[k] = v[r].im;
-        }
+        if (k <= N / 2) hab[k] = v[r];
     }
     colpass_nyquist<T, LOG2N>(a, pair);
 }
@@ -425,7 +429,7 @@ template <typename T>
 struct SampleArgs {
     const cpx<T>* S;             // complex spectrum (general path)
     long long rs, cs;            // element (row, col) at row*rs + col*cs
-    const T* H;                  // real half spectrum (symmetric path), column stride ldH
+    const T* H;                  // real half spectrum (symmetric path), h_index layout
     int ldH;
     int nrow, ncol;
     T v_origin, duv;
@@ -522,9 +526,8 @@ __device__ __forceinline__ cpx<T> sample_point_sym(const SampleArgs<T>& a, T u0,
     const T q = g.q, t = g.t;
     const int half = a.nrow / 2;
     const int m0 = abs(g.fl_v - half), m1 = abs(g.fv1 - half);
-    const T* c0 = a.H + size_t(g.fl_u) * a.ldH;
-    const T* c1 = a.H + size_t(g.fu1) * a.ldH;
-    const T y0 = c0[m0], y1 = c0[m1], y3 = c1[m0], y2 = c1[m1];
+    const T y0 = a.H[h_index(g.fl_u, m0, a.ldH)], y1 = a.H[h_index(g.fl_u, m1, a.ldH)];
+    const T y3 = a.H[h_index(g.fu1, m0, a.ldH)], y2 = a.H[h_index(g.fu1, m1, a.ldH)];
     const T z = (t * (q * ((y0 + y2) - (y1 + y3))) + (t * (y1 - y0) + q * (y3 - y0))) + y0;
     const T a0 = fabs(y0), a1 = fabs(y1), a2 = fabs(y2), a3 = fabs(y3);
     T amp = a0;
