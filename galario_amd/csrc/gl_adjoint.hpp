@@ -56,7 +56,7 @@ constexpr int GHOT_ELEMS = GREP * GHOT * GHOT;
 
 template <typename T>
 struct GradArgs {
-    double* Hh;          // adjoint of H, same layout (column stride ldH), zero on entry
+    double* Hh;          // adjoint of H, same layout (h_index, ldH), zero on entry
     double* hot;         // GREP x GHOT x GHOT replicas [rep][col][row], zero on entry
     double* partials;    // 4 streams of per-workgroup partials: chi2, dPA, ddRA, ddDec
     int ldp;             // stride between the streams (>= number of workgroups)
@@ -146,8 +146,8 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
         }
     }
     if (oob && a.flag) atomicOr(a.flag, 1u);
-    // bounding box of the workgroup's cells
-    int b0 = live ? cu0 : INT_MAX, b1 = live ? cu1 : -1;
+    // bounding box of the workgroup's cells, columns in pairs (the window follows h_index)
+    int b0 = live ? (cu0 >> 1) : INT_MAX, b1 = live ? (cu1 >> 1) : -1;
     int b2 = live ? min(cm0, cm1) : INT_MAX, b3 = live ? max(cm0, cm1) : -1;
     b0 = wave_min(b0);
     b1 = wave_max(b1);
@@ -160,13 +160,13 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
         box[wv][3] = b3;
     }
     __syncthreads();
-    const int umin = min(min(box[0][0], box[1][0]), min(box[2][0], box[3][0]));
-    const int umax = max(max(box[0][1], box[1][1]), max(box[2][1], box[3][1]));
+    const int pmin = min(min(box[0][0], box[1][0]), min(box[2][0], box[3][0]));
+    const int pmax = max(max(box[0][1], box[1][1]), max(box[2][1], box[3][1]));
     const int mmin = min(min(box[0][2], box[1][2]), min(box[2][2], box[3][2]));
     const int mmax = max(max(box[0][3], box[1][3]), max(box[2][3], box[3][3]));
-    const int wu = umax - umin + 1, wm = mmax - mmin + 1;
-    const bool use_win = umax >= 0 && wu * wm <= GWIN;     // (wu, wm <= 2^13, no overflow)
-    const int nwin = use_win ? wu * wm : 0;
+    const int wp = pmax - pmin + 1, wm = mmax - mmin + 1, w2 = 2 * wm;
+    const bool use_win = pmax >= 0 && wp * w2 <= GWIN;     // (wp, wm <= 2^13, no overflow)
+    const int nwin = use_win ? wp * w2 : 0;
     for (int j = tid; j < nwin; j += 256) win[j] = 0.0;
     __syncthreads();
     if (use_win) {
@@ -176,26 +176,27 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
             for (int k = 0; k < 4; ++k) {
                 if (gk[k] == 0.0) continue;
                 if (cu[k] < GHOT && cm[k] < GHOT) lds_add_f64(&hot[cu[k] * GHOT + cm[k]], gk[k]);
-                else lds_add_f64(&win[(cu[k] - umin) * wm + (cm[k] - mmin)], gk[k]);
+                else lds_add_f64(&win[((cu[k] >> 1) - pmin) * w2 + 2 * (cm[k] - mmin) + (cu[k] & 1)], gk[k]);
             }
         }
     } else {
-        // Straight to Hh.  The FP64 atomics are served per 64-byte line and wave instruction, so lane
-        // pairs issue the two corners of one column of one point together (adjacent rows m0, m1): in
-        // round r, lane l carries point (l >> 1) + 32 (r & 1) of its wave, column r >> 1, row m0 or m1.
+        // Straight to Hh.  The FP64 atomics are served per 64-byte line and wave instruction, and
+        // a point's four corners lie in one 32-byte run of Hh when its lower column is even
+        // (h_index), so lane quads issue them together: in round r, lane l carries corner l & 3
+        // (gk order) of point (l >> 2) + 16 r of its wave.
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int src = (lane >> 1) + 32 * (r & 1);
-            const bool hi = lane & 1;
-            const int col = __shfl((r >> 1) ? cu1 : cu0, src, 64);
+            const int src = (lane >> 2) + 16 * r, corner = lane & 3;
+            const int c_lo = __shfl(cu0, src, 64), c_hi = __shfl(cu1, src, 64);
             const int m_a = __shfl(cm0, src, 64), m_b = __shfl(cm1, src, 64);
-            const double v_a = __shfl((r >> 1) ? gk[2] : gk[0], src, 64);
-            const double v_b = __shfl((r >> 1) ? gk[3] : gk[1], src, 64);
-            const int m = hi ? m_b : m_a;
-            const double x = hi ? v_b : v_a;
+            const double g0 = __shfl(gk[0], src, 64), g1 = __shfl(gk[1], src, 64);
+            const double g2 = __shfl(gk[2], src, 64), g3 = __shfl(gk[3], src, 64);
+            const int col = (corner >> 1) ? c_hi : c_lo;
+            const int m = (corner & 1) ? m_b : m_a;
+            const double x = corner == 0 ? g0 : corner == 1 ? g1 : corner == 2 ? g2 : g3;
             if (x != 0.0) {
                 if (col < GHOT && m < GHOT) lds_add_f64(&hot[col * GHOT + m], x);
-                else atomic_add_f64(g.Hh + size_t(col) * a.ldH + m, x);
+                else atomic_add_f64(g.Hh + h_index(col, m, a.ldH), x);
             }
         }
     }
@@ -206,15 +207,17 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
             const double x = hot[j];
             if (x != 0.0) atomic_add_f64(rep + j, x);
         }
-        // window cell j = c * wm + mm; c from a float reciprocal (exact after one correction, j < 2^24)
-        const float inv_wm = 1.0f / float(wm);
+        // window cell j = cp * w2 + 2 mm + (column & 1) is Hh offset ((pmin + cp) ldH + mmin) * 2 + j - cp * w2
+        // (consecutive j, consecutive addresses); cp from a float reciprocal (exact after one
+        // correction, j < 2^24)
+        const float inv_w2 = 1.0f / float(w2);
         for (int j = tid; j < nwin; j += 256) {
             const double x = win[j];
             if (x != 0.0) {
-                int c = int(float(j) * inv_wm);
-                c += (c + 1) * wm <= j;
-                c -= c * wm > j;
-                atomic_add_f64(g.Hh + size_t(umin + c) * a.ldH + (mmin + (j - c * wm)), x);
+                int c = int(float(j) * inv_w2);
+                c += (c + 1) * w2 <= j;
+                c -= c * w2 > j;
+                atomic_add_f64(g.Hh + (size_t(pmin + c) * a.ldH + mmin) * 2 + (j - c * w2), x);
             }
         }
     }
@@ -314,8 +317,7 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
     const int la = 2 * pair, lb = la + 1;
     const bool has_b = lb < C;
     const int tid = threadIdx.x;
-    double* ha = a.Hh + size_t(la) * a.ldH;
-    double* hb = ha + a.ldH;
+    cpx<double>* hab = reinterpret_cast<cpx<double>*>(a.Hh + h_index(la, 0, a.ldH));   // columns la, lb (h_index)
     const bool hot_cols = la < GHOT;   // GHOT is even: lb < GHOT too
     const auto hot_sum = [&](int l, int p) {
         double s = 0.0;
@@ -326,7 +328,8 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
     F::run(v, lds, tid, N / 2, a.tw, [&](int k) -> cpx<T> {
         const int p = k <= N / 2 ? k : N - k;
         const double wk = ext_weight(k, N);
-        double xa = ha[p], xb = has_b ? hb[p] : 0.0;
+        const cpx<double> x = hab[p];
+        double xa = x.re, xb = has_b ? x.im : 0.0;
         if (hot_cols && p < GHOT) {
             xa += hot_sum(la, p);
             if (has_b) xb += hot_sum(lb, p);
@@ -334,10 +337,7 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
         return cpx<T>{T(wk * xa), T(wk * xb)};
     }, a.Rs < NT);        // outputs m <= Rs' only
     // all reads of Hh are done (the FFT synchronised after its first pass): clear it for the next call
-    for (int p = tid; p <= N / 2; p += NT) {
-        ha[p] = 0.0;
-        if (has_b) hb[p] = 0.0;
-    }
+    for (int p = tid; p <= N / 2; p += NT) hab[p] = cpx<double>{0.0, 0.0};   // (slot lb of the last pair: padding)
     if (hot_cols)
         for (int j = tid; j < GREP * GHOT; j += NT) {
             const int r = j / GHOT, p = j % GHOT;
