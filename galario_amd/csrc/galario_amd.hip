@@ -1034,7 +1034,7 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         // adjoint passes; workgroup 0 of the adjoint column pass also sums the partials into chi2 and
         // the PA, dRA, dDec gradients (the order of k_sum_partials)
         const int ldTh = (g.C + 1 + 7) & ~7;
-        T* Th = w.Th.get<T>(size_t(2 * g.pairs_r) * ldTh);   // ts_index blocks: row pairs
+        T* Th = w.Th.get<T>(size_t(g.nrows) * ldTh);
         const cpx<T>* tw = twiddles<T>(nxy);
         const Dst4 dst{{chi2_d, grad_geom + 1, grad_geom + 2, grad_geom + 3}};
         const int nfft = 8 * ((g.pairs_c + 7) / 8);
@@ -1094,7 +1094,6 @@ struct BatchPlan : BatchPlanBase {
     int ldPart = 1;
     std::vector<BatchGroup<T>> groups;
     size_t sTs = 0, sH = 0, sHh = 0, hot_off = 0, sTh = 0;
-    int ldTh = 0;                       // reals per row of every group's adjoint rows (ts_index)
     int slots_ts = 0, slots_grad = 0;   // slots allocated (forward buffers, gradient buffers)
     bool Hh_zero = false;
     DevBuf wp, I, geom, part, Ts, H, Hh, Th, out_sets, out_tot, gI, ggeom, flag;
@@ -1253,7 +1252,7 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, 
             }
             if (grad) {
                 AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0, nullptr, 0, Dst4{}, ablocks};
-                aca.ldTh = P.ldTh;
+                aca.ldTh = int(P.sTh / size_t(g.nrows));
                 AdjRowArgs<T> ara{G.sw, nullptr, aca.ldTh, G.N, g.Rs, g.nrows, tw, 0.0, nullptr, nullptr};
 #define GL_CALL(LL)                                                                                              \
     hipLaunchKernelGGL((k_adjcol_sym_b<T, LL>), dim3(ablocks, nw), dim3(FftShape<LL>::NT), 0, s, aca, b);        \
@@ -1338,11 +1337,10 @@ int batch_create(int nsets, const BatchSet<T>* sets, int nr, T Rmin, T dR, int m
         }
         P->hot_off = (maxHh + 31) & ~size_t(31);
         P->sHh = P->hot_off + size_t(GHOT_ELEMS);
-        // adjoint rows: row pairs x ldTh per group (ts_index blocks), one row length for all
+        // adjoint rows: nrows x ldTh per group, one stride for all
         size_t sTh = 0;
-        for (const BatchGroup<T>& G : P->groups) sTh = std::max(sTh, size_t(2 * G.g.pairs_r) * maxTh);
+        for (const BatchGroup<T>& G : P->groups) sTh = std::max(sTh, size_t(G.g.nrows) * maxTh);
         P->sTh = sTh;
-        P->ldTh = int(maxTh);
         // parameters, walker scalars, partials, outputs
         P->I.template get<T>(size_t(max_w) * nr);
         P->geom.template get<T>(size_t(max_w) * 4);

@@ -257,7 +257,7 @@ struct AdjColArgs {
     int ldH;
     int N, Rs;           // Rs': outputs m = 0..Rs'
     const cpx<T>* tw;
-    T* Th;               // rows m = 0..Rs' of ldTh reals (a multiple of 8), in ts_index blocks
+    T* Th;               // row-major [Rs'+1][ldTh]
     int ldTh;
     const double* part;  // optional: 4 streams of np partials (stride np), each summed into dst by one of
     int np;              // 8 extra workgroups after the nfft FFT workgroups
@@ -349,7 +349,7 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
         const int m = tid + r * NT;
         if (r * NT <= a.Rs && m <= a.Rs) {
             const T cm = T(mult(m, N));
-            T* o = a.Th + ts_index(m, la, a.ldTh);     // columns la, la + 1: adjacent (la even)
+            T* o = a.Th + size_t(m) * a.ldTh + la;
             o[0] = cm * v[r].re;
             if (has_b) o[1] = cm * v[r].im;
         }
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym(AdjColArgs<T
 template <typename T>
 struct AdjRowArgs {
     SweepPrm<T> sw;
-    const T* Th;         // ts_index blocks, ldTh reals per row
+    const T* Th;
     int ldTh;
     int N, Rs, nrows;    // pixels j = 0..Rs', rows m = 0..nrows-1
     const cpx<T>* tw;
@@ -437,13 +437,13 @@ __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk)
     for (int k = tid; k < a.sw.nr; k += NT) gIl[k] = 0.0;   // ordered before use by the FFT's barriers
     const int ma = 2 * blk, mb = ma + 1;
     const bool hb = mb < a.nrows;
-    const T* rab = a.Th + ts_index(ma, 0, a.ldTh);     // rows ma, mb: one block row (ts_index)
+    const T* ra = a.Th + size_t(ma) * a.ldTh;
+    const T* rb = ra + a.ldTh;
     cpx<T> v[E];
     F::run(v, lds, tid, N / 2, a.tw, [&](int k) -> cpx<T> {
         const int l = k <= N / 2 ? k : N - k;
         const T wk = T(ext_weight(k, N));
-        const T* blk4 = rab + ((l >> 2) << 3) + (l & 3);   // row ma; row mb 4 reals on
-        return cpx<T>{wk * blk4[0], hb ? wk * blk4[4] : T(0)};
+        return cpx<T>{wk * ra[l], hb ? wk * rb[l] : T(0)};
     }, a.Rs < NT);        // pixels j <= Rs' only
     double ginc = 0.0;
 #pragma unroll
