@@ -425,10 +425,11 @@ __device__ __noinline__ void central_adjoint(double gz, const SweepPrm<T>& p, do
 
 constexpr int ADJROW_LDS_NR = 2048;   // profile bins accumulated in LDS by the adjoint row pass
 
-template <typename T, int LOG2N>
+// SPLIT: as in rowpass_sym_body (the batched form uses the half-LDS exchange).
+template <typename T, int LOG2N, bool SPLIT = false>
 __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk) {
     using S = FftShape<LOG2N>;
-    using F = BlockFft<T, LOG2N, false, false>;     // few workgroups: the lower-latency exchange
+    using F = BlockFft<T, LOG2N, false, SPLIT>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
     __shared__ T lds[F::LDS_REALS];
     __shared__ double red[16];

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_rowpass_sym_b(SymRowArg
     a.Ts = b.Ts + slot * b.sTs;
     a.zero = nullptr;
     a.zero1 = nullptr;
-    rowpass_sym_body<T, LOG2N>(a, blockIdx.x);
+    rowpass_sym_body<T, LOG2N, true>(a, blockIdx.x);   // many walkers: half-LDS exchange
 }
 
 template <typename T, int LOG2N>
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjrow_sym_b(AdjRowArgs
     a.tan_inc = b.wp[w].tan_inc;
     a.gI = b.gI + size_t(w) * b.nr;
     a.ginc = b.ggeom + size_t(w) * 4;
-    adjrow_sym_body<T, LOG2N>(a, blockIdx.x);
+    adjrow_sym_body<T, LOG2N, true>(a, blockIdx.x);    // many walkers: half-LDS exchange
 }
 
 constexpr int BATCH_MAX_SETS = 32;

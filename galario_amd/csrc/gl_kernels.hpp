@@ -319,10 +319,13 @@ struct SymRowArgs {
     double* zero1;
 };
 
-template <typename T, int LOG2N>
+// SPLIT: the half-LDS exchange (more workgroups per CU: the batched form's many
+// walkers); otherwise both parts per round trip (lower latency: one call's
+// ~Rs'/2 workgroups).
+template <typename T, int LOG2N, bool SPLIT = false>
 __device__ __forceinline__ void rowpass_sym_body(const SymRowArgs<T>& a, int blk) {
     using S = FftShape<LOG2N>;
-    using F = BlockFft<T, LOG2N, false, false>;     // few workgroups: the lower-latency exchange
+    using F = BlockFft<T, LOG2N, false, SPLIT>;
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
     __shared__ T lds[F::LDS_REALS];
