@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, 'libgalario_amd.so')
 OK, ERR_INVALID, ERR_ALLOC, ERR_RUNTIME = 0, 1, 2, 3
 
 _c_int = ctypes.c_int
+_c_double = ctypes.c_double
 _vp = ctypes.c_void_p
 _ip = ctypes.POINTER(ctypes.c_int)
 
@@ -41,13 +42,15 @@ def _preload_torch_hip_runtime() -> None:
 class DatasetF64(ctypes.Structure):
     """galario_dataset_f64 (include/galario_amd.h)."""
     _fields_ = [('nxy', ctypes.c_int), ('dxy', ctypes.c_double), ('duv', ctypes.c_double), ('nd', ctypes.c_int),
-                ('u', _vp), ('v', _vp), ('vis_obs_re', _vp), ('vis_obs_im', _vp), ('weights', _vp)]
+                ('u', _vp), ('v', _vp), ('vis_obs_re', _vp), ('vis_obs_im', _vp), ('weights', _vp),
+                ('uvmax', ctypes.c_double)]
 
 
 class DatasetF32(ctypes.Structure):
     """galario_dataset_f32 (include/galario_amd.h)."""
     _fields_ = [('nxy', ctypes.c_int), ('dxy', ctypes.c_float), ('duv', ctypes.c_float), ('nd', ctypes.c_int),
-                ('u', _vp), ('v', _vp), ('vis_obs_re', _vp), ('vis_obs_im', _vp), ('weights', _vp)]
+                ('u', _vp), ('v', _vp), ('vis_obs_re', _vp), ('vis_obs_im', _vp), ('weights', _vp),
+                ('uvmax', ctypes.c_double)]
 
 
 BATCH_EAGER = 1
@@ -103,16 +106,17 @@ def _load():
             'interpolate': [_c_int, _c_int, _vp, R, _c_int, _vp, _vp, R, _vp],
             'apply_phase_sampled': [R, R, _c_int, _vp, _vp, _vp],
             'reduce_chi2': [_c_int, _vp, _vp, _vp, _vp, _vp],
+            # the device-resident forms end with uvmax (double; <= 0: no uv extent)
             'chi2_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp, _vp,
-                                 _vp, _c_int, _vp],
+                                 _vp, _c_int, _vp, _c_double],
             'chi2_profile_grad_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp,
-                                      _vp, _vp, _c_int, _vp, _vp, _vp],
+                                      _vp, _vp, _c_int, _vp, _vp, _vp, _c_double],
             'sample_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _c_int,
-                                   _vp, _vp],
+                                   _vp, _vp, _c_double],
             'sample_profile_multi_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, _c_int, _vp, _c_int, _vp, _vp,
-                                         _c_int, _vp, _vp],
+                                         _c_int, _vp, _vp, _c_double],
             'chi2_profile_multi_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, _c_int, _vp, _c_int, _vp, _vp,
-                                       _vp, _vp, _vp, _c_int, _vp],
+                                       _vp, _vp, _vp, _c_int, _vp, _c_double],
         }
         for name, args in sig.items():
             f = getattr(lib, f'galario_{name}_{suf}')

@@ -389,6 +389,27 @@ cpx<T>* build_spectrum(Workspace& w, hipStream_t s, RowArgs<T> ra, int N, int Rs
     return S_out;
 }
 
+// uv extent of a dataset whose points lie within uvmax of the origin (uv
+// units; <= 0: not given).  Rotation keeps |(u, v)|, so at any PA a point's
+// grid position lies within rho = uvmax / duv cells of the origin (rho2: with
+// slack for the rotation's rounding) and its interpolation corners within
+// rho + sqrt(2): the half spectrum is stored, and its adjoint read, only
+// within rc = rho + 3 cells.
+struct Extent {
+    double rho2 = 0.0, rc2 = 0.0;
+};
+
+template <typename T>
+Extent uv_extent(double uvmax, double duv) {
+    Extent e;
+    if (!(uvmax > 0.0) || !std::isfinite(uvmax) || !(duv > 0.0)) return e;
+    const double rel = sizeof(T) == 8 ? 1e-12 : 1e-5;
+    const double rho = uvmax / duv * (1.0 + rel) + 1e-6;
+    e.rho2 = rho * rho;
+    e.rc2 = (rho + 3.0) * (rho + 3.0);
+    return e;
+}
+
 // Real, even half spectrum H of a swept profile (see k_rowpass_sym); returns
 // false when the workgroup FFT does not cover N (the caller then uses the
 // general complex path).
@@ -415,7 +436,8 @@ SymGeom sym_geom(int L, int N, int rmax) {
 // accumulators of the same call).
 template <typename T>
 bool build_spectrum_sym(Workspace& w, hipStream_t s, const SweepPrm<T>& sw, int N, int rmax, const T** H_out,
-                        int* ldH_out, double* zero = nullptr, int nzero = 0, double* zero1 = nullptr) {
+                        int* ldH_out, double* zero = nullptr, int nzero = 0, double* zero1 = nullptr,
+                        double rc2 = 0.0) {
     const int L = fast_log2<T>(N);
     if (L < 0) return false;
     const SymGeom g = sym_geom(L, N, rmax);
@@ -423,7 +445,7 @@ bool build_spectrum_sym(Workspace& w, hipStream_t s, const SweepPrm<T>& sw, int 
     T* H = w.S.get<T>(size_t(g.C + 1) * g.ldH);
     const cpx<T>* tw = twiddles<T>(N);
     SymRowArgs<T> ra{sw, N, g.Rs, g.nrows, g.ldT, tw, Ts, zero, nzero, zero1};
-    SymColArgs<T> ca{Ts, N, g.Rs, g.ldT, tw, H, g.ldH};
+    SymColArgs<T> ca{Ts, N, g.Rs, g.ldT, tw, H, g.ldH, rc2};
     t_timer.mark(ST_ROW, s);
     for (int r = t_timer.reps(ST_ROW); r > 0; --r) launch_rowpass_sym<T>(L, g.pairs_r, s, ra);
     t_timer.mark(ST_COL, s);
@@ -569,14 +591,16 @@ void launch_vis(hipStream_t s, SampleArgs<T> a, cpx<T>* vis) {
 // general complex spectrum.
 template <typename T>
 SampleArgs<T> profile_spectrum(Workspace& w, hipStream_t s, ProfileSetup<T>& ps, const SampleSetup<T>& ss, int N,
-                               T duv, int nd, const T* u, const T* v) {
+                               T duv, int nd, const T* u, const T* v, double uvmax = 0.0) {
     const int C = N / 2 + 1;
     const T* H = nullptr;
     int ldH = 0;
-    if (build_spectrum_sym<T>(w, s, ps.sw, N, ps.rmax, &H, &ldH)) {
+    const Extent ex = uv_extent<T>(uvmax, double(duv));
+    if (build_spectrum_sym<T>(w, s, ps.sw, N, ps.rmax, &H, &ldH, nullptr, 0, nullptr, ex.rc2)) {
         SampleArgs<T> sa = sample_args<T>(nullptr, 0, 0, N, C, T(1), duv, ss, nd, u, v);
         sa.H = H;
         sa.ldH = ldH;
+        sa.rho2 = ex.rho2;
         return sa;
     }
     RowArgs<T> ra{};
@@ -905,7 +929,7 @@ int reduce_chi2_host(int nd, const T* re, const T* im, const void* vis, const T*
 template <typename T>
 int chi2_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec, T duv,
                      T PA, int nd, const T* u, const T* v, const T* re, const T* im, const T* wt, int check_central,
-                     double* chi2_d) {
+                     double* chi2_d, double uvmax) {
     return guard([&] {
         check_nd(nd);
         ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
@@ -913,7 +937,7 @@ int chi2_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int 
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
-        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, u, v);
+        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, u, v, uvmax);
         sa.flag = w.flag_dev.get<unsigned>(1);
         launch_chi2<T>(w, s, sa, re, im, wt, chi2_d, nullptr);
         t_timer.end_call(s);
@@ -922,7 +946,8 @@ int chi2_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int 
 
 template <typename T>
 int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec, T duv,
-                       T PA, int nd, const T* u, const T* v, int check_central, const int* perm, void* vis_d) {
+                       T PA, int nd, const T* u, const T* v, int check_central, const int* perm, void* vis_d,
+                       double uvmax) {
     return guard([&] {
         check_nd(nd);
         ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
@@ -930,7 +955,7 @@ int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, in
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
-        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, u, v);
+        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, u, v, uvmax);
         sa.flag = w.flag_dev.get<unsigned>(1);
         sa.perm = perm;
         launch_vis(s, sa, static_cast<cpx<T>*>(vis_d));
@@ -944,7 +969,7 @@ int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, in
 template <typename T>
 int profile_multisrc_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T duv, T PA, int nsrc,
                          const T* src, int nd, const T* u, const T* v, const T* re, const T* im, const T* wt,
-                         int check_central, const int* perm, void* vis_d, double* chi2_d) {
+                         int check_central, const int* perm, void* vis_d, double* chi2_d, double uvmax) {
     return guard([&] {
         check_nd(nd);
         if (nsrc < 0) invalid("multiple sources", "nsrc = " + std::to_string(nsrc) + " is negative");
@@ -954,7 +979,7 @@ int profile_multisrc_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, 
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
-        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, u, v);
+        SampleArgs<T> sa = profile_spectrum<T>(w, s, ps, ss, nxy, duv, nd, u, v, uvmax);
         sa.flag = w.flag_dev.get<unsigned>(1);
         const unsigned nb = blocks_for(size_t(nd), 256);
         if (chi2_d) {
@@ -983,7 +1008,7 @@ int profile_multisrc_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, 
 template <typename T>
 int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec,
                           T duv, T PA, int nd, const T* u, const T* v, const T* re, const T* im, const T* wt,
-                          int check_central, double* chi2_d, double* grad_I, double* grad_geom) {
+                          int check_central, double* chi2_d, double* grad_I, double* grad_geom, double uvmax) {
     return guard([&] {
         check_nd(nd);
         ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
@@ -1016,10 +1041,12 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         const T* H = nullptr;
         int ldH = 0;
         // the forward row pass clears the accumulators grad_I[0..nr) and grad_geom[0] (inc)
-        build_spectrum_sym<T>(w, s, ps.sw, nxy, ps.rmax, &H, &ldH, grad_I, nr, grad_geom);
+        const Extent ex = uv_extent<T>(uvmax, double(duv));
+        build_spectrum_sym<T>(w, s, ps.sw, nxy, ps.rmax, &H, &ldH, grad_I, nr, grad_geom, ex.rc2);
         SampleArgs<T> sa = sample_args<T>(nullptr, 0, 0, nxy, g.C, T(1), duv, ss, nd, u, v);
         sa.H = H;
         sa.ldH = ldH;
+        sa.rho2 = ex.rho2;
         sa.re = re;
         sa.im = im;
         sa.w = wt;
@@ -1038,7 +1065,7 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         const cpx<T>* tw = twiddles<T>(nxy);
         const Dst4 dst{{chi2_d, grad_geom + 1, grad_geom + 2, grad_geom + 3}};
         const int nfft = 8 * ((g.pairs_c + 7) / 8);
-        AdjColArgs<T> aca{Hh, hot, g.ldH, nxy, g.Rs, tw, Th, ldTh, part, int(nb), dst, nfft};
+        AdjColArgs<T> aca{Hh, hot, g.ldH, nxy, g.Rs, tw, Th, ldTh, part, int(nb), dst, nfft, ex.rc2};
         t_timer.mark(ST_ADJCOL, s);
         launch_adjcol_sym<T>(L, nfft + 8, s, aca);   // + 8 workgroups for the partial sums
         HC(hipGetLastError());
@@ -1074,6 +1101,7 @@ struct BatchSet {
     T dxy, duv;
     int nd;
     const T *u, *v, *re, *im, *w;
+    double uvmax;          // <= 0: not given
 };
 
 template <typename T>
@@ -1083,6 +1111,8 @@ struct BatchGroup {
     SymGeom g;
     SweepPrm<T> sw;        // shared scalars (I and cos_inc are per walker)
     std::vector<int> sets;
+    Extent ex;             // the widest extent of its datasets (none if one has none)
+    bool ex_all = true;    // every dataset so far gives its extent
 };
 
 template <typename T>
@@ -1220,7 +1250,7 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, 
             const int L = g.L;
             const cpx<T>* tw = twiddles<T>(G.N);
             SymRowArgs<T> ra{G.sw, G.N, g.Rs, g.nrows, g.ldT, tw, nullptr, nullptr, 0, nullptr};
-            SymColArgs<T> ca{nullptr, G.N, g.Rs, g.ldT, tw, nullptr, g.ldH};
+            SymColArgs<T> ca{nullptr, G.N, g.Rs, g.ldT, tw, nullptr, g.ldH, G.ex.rc2};
             const int cblocks = G.N / 4;                      // column pairs 0..N/2-1, column N/2 inside
             const int ablocks = 8 * ((g.pairs_c + 7) / 8);    // adjoint: (C+1)/2 pairs
 #define GL_CALL(LL)                                                                                              \
@@ -1244,6 +1274,7 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, 
                 a.im = d.im;
                 a.w = d.w;
                 a.flag = flag;
+                a.rho2 = G.ex.rho2;
                 b.partOff = P.off[si];
                 if (grad)
                     hipLaunchKernelGGL((k_sample_grad_sym_b<T>), dim3(P.nb[si], nw), dim3(256), 0, s, a, b);
@@ -1251,7 +1282,8 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, 
                     launch_sample_multi<T>(kw, dim3(P.nb[si], (nw + kw - 1) / kw), s, a, b, nw);
             }
             if (grad) {
-                AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0, nullptr, 0, Dst4{}, ablocks};
+                AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0, nullptr, 0, Dst4{}, ablocks,
+                                  G.ex.rc2};
                 aca.ldTh = int(P.sTh / size_t(g.nrows));
                 AdjRowArgs<T> ara{G.sw, nullptr, aca.ldTh, G.N, g.Rs, g.nrows, tw, 0.0, nullptr, nullptr};
 #define GL_CALL(LL)                                                                                              \
@@ -1317,7 +1349,12 @@ int batch_create(int nsets, const BatchSet<T>* sets, int nr, T Rmin, T dR, int m
                 P->groups.push_back(G);
                 gi = int(P->groups.size()) - 1;
             }
-            P->groups[size_t(gi)].sets.push_back(i);
+            BatchGroup<T>& Gi = P->groups[size_t(gi)];
+            Gi.sets.push_back(i);
+            const Extent e = uv_extent<T>(d.uvmax, double(d.duv));
+            Gi.ex_all = Gi.ex_all && e.rc2 > 0.0;
+            Gi.ex.rho2 = Gi.ex_all ? std::max(Gi.ex.rho2, e.rho2) : 0.0;
+            Gi.ex.rc2 = Gi.ex_all ? std::max(Gi.ex.rc2, e.rc2) : 0.0;
             P->sets.push_back(d);
             const int nb = int(blocks_for(size_t(d.nd), 256));
             P->off.push_back(int(blocks));
@@ -1585,15 +1622,15 @@ void galario_free(void* ptr) { std::free(ptr); }
     }                                                                                                             \
     int galario_chi2_profile_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc,    \
                                        T dRA, T dDec, T duv, T PA, int nd, const T* u, const T* v, const T* re,  \
-                                       const T* im, const T* w, int cc, double* chi2_d) {                        \
+                                       const T* im, const T* w, int cc, double* chi2_d, double uvmax) {          \
         return chi2_profile_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v, re, im, \
-                                   w, cc, chi2_d);                                                               \
+                                   w, cc, chi2_d, uvmax);                                                        \
     }                                                                                                             \
     int galario_sample_profile_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc,  \
                                          T dRA, T dDec, T duv, T PA, int nd, const T* u, const T* v, int cc,     \
-                                         const int* perm, void* vis_d) {                                         \
+                                         const int* perm, void* vis_d, double uvmax) {                           \
         return sample_profile_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v, cc,   \
-                                     perm, vis_d);                                                               \
+                                     perm, vis_d, uvmax);                                                        \
     }
 
 GL_BOTH(GL_MAIN)
@@ -1602,9 +1639,9 @@ GL_BOTH(GL_MAIN)
     int galario_chi2_profile_grad_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy,       \
                                             T inc, T dRA, T dDec, T duv, T PA, int nd, const T* u, const T* v,    \
                                             const T* re, const T* im, const T* w, int cc, double* chi2_d,         \
-                                            double* grad_I, double* grad_geom) {                                  \
+                                            double* grad_I, double* grad_geom, double uvmax) {                    \
         return chi2_profile_grad_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v, re, \
-                                        im, w, cc, chi2_d, grad_I, grad_geom);                                    \
+                                        im, w, cc, chi2_d, grad_I, grad_geom, uvmax);                             \
     }
 
 GL_BOTH(GL_GRAD)
@@ -1612,20 +1649,20 @@ GL_BOTH(GL_GRAD)
 #define GL_MULTI(SUF, T)                                                                                          \
     int galario_sample_profile_multi_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy,    \
                                                T inc, T duv, T PA, int nsrc, const T* src, int nd, const T* u,    \
-                                               const T* v, int cc, const int* perm, void* vis_d) {                \
+                                               const T* v, int cc, const int* perm, void* vis_d, double uvmax) {  \
         return profile_multisrc_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, duv, PA, nsrc, src, nd, u, v,      \
-                                       nullptr, nullptr, nullptr, cc, perm, vis_d, nullptr);                      \
+                                       nullptr, nullptr, nullptr, cc, perm, vis_d, nullptr, uvmax);               \
     }                                                                                                             \
     int galario_chi2_profile_multi_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy,      \
                                              T inc, T duv, T PA, int nsrc, const T* src, int nd, const T* u,      \
                                              const T* v, const T* re, const T* im, const T* w, int cc,            \
-                                             double* chi2_d) {                                                    \
+                                             double* chi2_d, double uvmax) {                                      \
         if (!chi2_d) {                                                                                            \
             t_err = "Error in multiple sources: null chi2 pointer";                                               \
             return GALARIO_ERR_INVALID;                                                                           \
         }                                                                                                         \
         return profile_multisrc_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, duv, PA, nsrc, src, nd, u, v, re,  \
-                                       im, w, cc, nullptr, nullptr, chi2_d);                                      \
+                                       im, w, cc, nullptr, nullptr, chi2_d, uvmax);                               \
     }
 
 GL_BOTH(GL_MULTI)
@@ -1637,7 +1674,8 @@ GL_BOTH(GL_MULTI)
         if (sets && nsets > 0 && nsets <= BATCH_MAX_SETS)                                                         \
             for (int i = 0; i < nsets; ++i)                                                                       \
                 v.push_back(BatchSet<T>{sets[i].nxy, sets[i].dxy, sets[i].duv, sets[i].nd, sets[i].u, sets[i].v, \
-                                        sets[i].vis_obs_re, sets[i].vis_obs_im, sets[i].weights});               \
+                                        sets[i].vis_obs_re, sets[i].vis_obs_im, sets[i].weights,                 \
+                                        sets[i].uvmax});                                                          \
         return batch_create<T>(nsets, v.empty() ? nullptr : v.data(), nr, Rmin, dR, max_walkers, check_central,  \
                                plan);                                                                             \
     }                                                                                                             \

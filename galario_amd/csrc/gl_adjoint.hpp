@@ -263,6 +263,7 @@ struct AdjColArgs {
     int np;              // 8 extra workgroups after the nfft FFT workgroups
     Dst4 dst;
     int nfft;
+    double rc2;          // > 0: Hh is zero beyond the store radius sqrt(rc2) (stored_rows)
 };
 
 // Deterministic sum *dst = sum_i p[i] of n partials in the order of
@@ -297,7 +298,9 @@ __device__ __forceinline__ void sum_partials_in_block(const double* __restrict__
 __device__ __forceinline__ double mult(int m, int N) { return (m == 0 || m == N / 2) ? 1.0 : 2.0; }
 __device__ __forceinline__ double ext_weight(int k, int N) { return (k == 0 || k == N / 2) ? 1.0 : 0.5; }
 
-template <typename T, int LOG2N>
+// PARTIALS: workgroups blk >= nfft sum the sample kernel's partial streams
+// (the batched form sums them in its own reduction kernel).
+template <typename T, int LOG2N, bool PARTIALS = true>
 __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk, int nblk) {
     using S = FftShape<LOG2N>;
     using F = BlockFft<T, LOG2N, false>;
@@ -306,7 +309,7 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
     constexpr int NPAIR = (C + 1) / 2;
     __shared__ T lds[S::LDS_ELEMS];
     static_assert(NT >= 64 && 1024 % NT == 0, "partial sums replay 1024 virtual threads");
-    if (blk >= a.nfft) {                    // extra workgroups: the partial sums of the sample kernel
+    if (PARTIALS && blk >= a.nfft) {        // extra workgroups: the partial sums of the sample kernel
         const int j = blk - a.nfft;
         __shared__ double red16[16];
         if (a.part && j < 4) sum_partials_in_block<NT>(a.part + size_t(j) * a.np, a.np, a.dst.d[j], red16);
@@ -319,6 +322,17 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
     const int tid = threadIdx.x;
     cpx<double>* hab = reinterpret_cast<cpx<double>*>(a.Hh + h_index(la, 0, a.ldH));   // columns la, lb (h_index)
     const bool hot_cols = la < GHOT;   // GHOT is even: lb < GHOT too
+    // rows the sample kernel can have touched (column lb: no more); beyond them Hh, and the
+    // replicas of the hot tile, hold zeros
+    const int kc = stored_rows(a.rc2, la, N);
+    if (kc < 0) {                      // no sample reaches these columns: their adjoint rows are zero
+        for (int m = tid; m <= a.Rs; m += NT) {
+            T* o = a.Th + size_t(m) * a.ldTh + la;
+            o[0] = T(0);
+            if (has_b) o[1] = T(0);
+        }
+        return;
+    }
     const auto hot_sum = [&](int l, int p) {
         double s = 0.0;
         for (int r = 0; r < GREP; ++r) s += a.hot[(r * GHOT + l) * GHOT + p];
@@ -328,7 +342,7 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
     F::run(v, lds, tid, N / 2, a.tw, [&](int k) -> cpx<T> {
         const int p = k <= N / 2 ? k : N - k;
         const double wk = ext_weight(k, N);
-        const cpx<double> x = hab[p];
+        const cpx<double> x = p <= kc ? hab[p] : cpx<double>{0.0, 0.0};
         double xa = x.re, xb = has_b ? x.im : 0.0;
         if (hot_cols && p < GHOT) {
             xa += hot_sum(la, p);
@@ -337,7 +351,7 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
         return cpx<T>{T(wk * xa), T(wk * xb)};
     }, a.Rs < NT);        // outputs m <= Rs' only
     // all reads of Hh are done (the FFT synchronised after its first pass): clear it for the next call
-    for (int p = tid; p <= N / 2; p += NT) hab[p] = cpx<double>{0.0, 0.0};   // (slot lb of the last pair: padding)
+    for (int p = tid; p <= kc; p += NT) hab[p] = cpx<double>{0.0, 0.0};   // (slot lb of the last pair: padding)
     if (hot_cols)
         for (int j = tid; j < GREP * GHOT; j += NT) {
             const int r = j / GHOT, p = j % GHOT;

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym_b(AdjColArgs
     a.Hh = b.Hh + slot * b.sHh;
     a.hot = a.Hh + b.hot_off;
     a.Th = b.Th + slot * b.sTh;
-    adjcol_sym_body<T, LOG2N>(a, blockIdx.x, gridDim.x);
+    adjcol_sym_body<T, LOG2N, false>(a, blockIdx.x, gridDim.x);   // partials: k_batch_reduce
 }
 
 template <typename T, int LOG2N>

@@ -368,7 +368,18 @@ struct SymColArgs {
     const cpx<T>* tw;
     T* H;
     int ldH;
+    double rc2;                  // > 0: store only cells within sqrt(rc2) (cells) of the origin (stored_rows)
 };
+
+// Highest row of column l of a half spectrum that a sample within the store
+// radius sqrt(rc2) can read (-1: none); N/2 when there is no radius (rc2 <= 0).
+__device__ __forceinline__ int stored_rows(double rc2, int l, int N) {
+    if (!(rc2 > 0.0)) return N / 2;
+    const double r = rc2 - double(l) * double(l);
+    if (r < 0.0) return -1;
+    const int k = int(sqrt(r));
+    return k < N / 2 ? k : N / 2;
+}
 
 // Column N/2 of H (the one column without a partner among the N/2 + 1):
 // H[N/2][k] = sum over the loaded rows n of Ts[|n|][N/2] cos(2 pi n k / N), i.e.
@@ -379,9 +390,10 @@ __device__ __forceinline__ void colpass_nyquist(const SymColArgs<T>& a, int blk)
     constexpr int N = 1 << LOG2N, NT = FftShape<LOG2N>::NT;
     constexpr int NW = NT / 64 > 0 ? NT / 64 : 1;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kmax = stored_rows(a.rc2, N / 2, N);
     for (int q = wv; q < 3; q += NW) {
         const int k = blk + q * (N / 4);
-        if (k > N / 2) break;
+        if (k > kmax) break;
         double acc = 0.0;
         for (int m = lane; m <= a.Rs; m += 64) {
             const double wm = (m == 0 || m == a.Rs) ? 1.0 : 2.0;
@@ -403,17 +415,20 @@ __device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk
     if (pair >= NPAIR) return;
     const int la = 2 * pair;
     const int tid = threadIdx.x;
-    // columns la, la + 1 of Ts: one aligned 16-byte pair per row m (la even)
-    cpx<T> v[E];
-    F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
-        const int m = n < N / 2 ? n : N - n;
-        return *reinterpret_cast<const cpx<T>*>(a.Ts + ts_index(m, la, a.ldT));
-    });
-    cpx<T>* hab = reinterpret_cast<cpx<T>*>(a.H + h_index(la, 0, a.ldH));   // columns la, la + 1
+    const int kc = stored_rows(a.rc2, la, N);   // rows a sample can read (column la + 1: no more)
+    if (kc >= 0) {                              // (else no sample reaches these columns)
+        // columns la, la + 1 of Ts: one aligned 16-byte pair per row m (la even)
+        cpx<T> v[E];
+        F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
+            const int m = n < N / 2 ? n : N - n;
+            return *reinterpret_cast<const cpx<T>*>(a.Ts + ts_index(m, la, a.ldT));
+        });
+        cpx<T>* hab = reinterpret_cast<cpx<T>*>(a.H + h_index(la, 0, a.ldH));   // columns la, la + 1
 #pragma unroll
-    for (int r = 0; r <= E / 2; ++r) {
-        const int k = tid + r * NT;
-        if (k <= N / 2) hab[k] = v[r];
+        for (int r = 0; r <= E / 2; ++r) {
+            const int k = tid + r * NT;
+            if (k <= kc) hab[k] = v[r];
+        }
     }
     colpass_nyquist<T, LOG2N>(a, pair);
 }
@@ -445,6 +460,7 @@ struct SampleArgs {
     const T* re; const T* im; const T* w;
     double* partials;            // MODE_CHI2: one per workgroup
     unsigned* flag;              // bit 0: a point fell outside the grid
+    double rho2;                 // > 0: squared uv extent in cells; points beyond it count as outside
 };
 
 // Grid coordinates of a (rotated) point; oob when it is not inside the grid
@@ -469,6 +485,10 @@ __device__ __forceinline__ GridPos<T> grid_pos(const SampleArgs<T>& a, T u0, T v
     g.sign_u = copysign(T(1), g.urot);
     const T indv = T(a.nrow / 2) + a.v_origin * g.sign_u * g.vrot / a.duv;
     g.oob = !(indu <= T(a.ncol - 1) && indv >= T(0) && indv <= T(a.nrow - 1));
+    if (a.rho2 > 0.0) {   // beyond the declared extent: the spectrum there may not have been stored
+        const double du = double(indu), dv = double(indv) - double(a.nrow / 2);
+        if (du * du + dv * dv > a.rho2) g.oob = true;
+    }
     g.fl_u = g.oob ? 0 : int(floor(indu));
     g.fl_v = g.oob ? 0 : int(floor(indv));
     g.q = indu - T(g.fl_u);

@@ -11,7 +11,9 @@ sorted by the Morton code of their Hermitian-folded position (|u|, sign(u) v),
 so that the threads of a wavefront gather neighbouring spectrum cells (a
 rotation by PA keeps neighbours together except across the new fold line).
 chi2 does not depend on the order; visibilities are returned in the caller's
-order (the kernel writes point i to perm[i]).
+order (the kernel writes point i to perm[i]).  A Dataset also records its uv
+extent (the largest sqrt(u^2 + v^2)); rotation keeps that radius, so its calls
+build the spectrum only within it.
 """
 from __future__ import annotations
 
@@ -50,8 +52,11 @@ def _check_tensors(dtype, *ts):
 
 
 def chi2_profile(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_im, vis_obs_w, dRA=0., dDec=0., PA=0.,
-                 inc=0., check_central_pixel=True, out=None):
-    """chi2Profile on device tensors; returns a 1-element float64 tensor (device)."""
+                 inc=0., check_central_pixel=True, out=None, uvmax=0.):
+    """chi2Profile on device tensors; returns a 1-element float64 tensor (device).
+
+    uvmax (> 0): a bound on sqrt(u^2 + v^2); the spectrum is then built only
+    within it (points beyond it count as outside the grid)."""
     dt = intensity.dtype
     dev = _check_tensors(dt, intensity, u, v, vis_obs_re, vis_obs_im, vis_obs_w)
     nd = u.numel()
@@ -63,13 +68,13 @@ def chi2_profile(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_im, vi
     with torch.cuda.device(dev):
         _check(f(_stream_ptr(dev), intensity.numel(), _p(intensity), Rmin, dR, dxy, int(nxy), inc, dRA, dDec,
                  1.0 / (dxy * nxy), PA, nd, _p(u), _p(v), _p(vis_obs_re), _p(vis_obs_im), _p(vis_obs_w),
-                 int(bool(check_central_pixel)), _p(out)))
+                 int(bool(check_central_pixel)), _p(out), float(uvmax)))
     return out
 
 
 def sample_profile(intensity, Rmin, dR, nxy, dxy, u, v, dRA=0., dDec=0., PA=0., inc=0., check_central_pixel=True,
-                   out=None, perm=None):
-    """sampleProfile on device tensors; returns a complex tensor (device)."""
+                   out=None, perm=None, uvmax=0.):
+    """sampleProfile on device tensors; returns a complex tensor (device).  uvmax as for chi2_profile."""
     dt = intensity.dtype
     dev = _check_tensors(dt, intensity, u, v)
     nd = u.numel()
@@ -81,7 +86,7 @@ def sample_profile(intensity, Rmin, dR, nxy, dxy, u, v, dRA=0., dDec=0., PA=0., 
     with torch.cuda.device(dev):
         _check(f(_stream_ptr(dev), intensity.numel(), _p(intensity), Rmin, dR, dxy, int(nxy), inc, dRA, dDec,
                  1.0 / (dxy * nxy), PA, nd, _p(u), _p(v), int(bool(check_central_pixel)),
-                 _p(perm) if perm is not None else None, _p(out)))
+                 _p(perm) if perm is not None else None, _p(out), float(uvmax)))
     return out
 
 
@@ -121,7 +126,7 @@ def _sources(sources, dt, dev):
 
 
 def sample_profile_multi(intensity, Rmin, dR, nxy, dxy, u, v, sources, PA=0., inc=0., check_central_pixel=True,
-                         out=None, perm=None):
+                         out=None, perm=None, uvmax=0.):
     """Visibilities of several sources with this profile, inc and PA: ``sources`` [K, 3] = (flux, dRA, dDec);
     sum_k flux_k sampleProfile(..., dRA_k, dDec_k) from one spectrum (complex device tensor)."""
     dt = intensity.dtype
@@ -134,12 +139,12 @@ def sample_profile_multi(intensity, Rmin, dR, nxy, dxy, u, v, sources, PA=0., in
     with torch.cuda.device(dev):
         _check(f(_stream_ptr(dev), intensity.numel(), _p(intensity), Rmin, dR, dxy, int(nxy), inc, 1.0 / (dxy * nxy),
                  PA, k, _p(src), nd, _p(u), _p(v), int(bool(check_central_pixel)),
-                 _p(perm) if perm is not None else None, _p(out)))
+                 _p(perm) if perm is not None else None, _p(out), float(uvmax)))
     return out
 
 
 def chi2_profile_multi(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_im, vis_obs_w, sources, PA=0., inc=0.,
-                       check_central_pixel=True, out=None):
+                       check_central_pixel=True, out=None, uvmax=0.):
     """chi2 of the multi-source model (see ``sample_profile_multi``); 1-element float64 device tensor."""
     dt = intensity.dtype
     dev = _check_tensors(dt, intensity, u, v, vis_obs_re, vis_obs_im, vis_obs_w)
@@ -150,7 +155,7 @@ def chi2_profile_multi(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_
     with torch.cuda.device(dev):
         _check(f(_stream_ptr(dev), intensity.numel(), _p(intensity), Rmin, dR, dxy, int(nxy), inc, 1.0 / (dxy * nxy),
                  PA, k, _p(src), u.numel(), _p(u), _p(v), _p(vis_obs_re), _p(vis_obs_im), _p(vis_obs_w),
-                 int(bool(check_central_pixel)), _p(out)))
+                 int(bool(check_central_pixel)), _p(out), float(uvmax)))
     return out
 
 
@@ -194,6 +199,9 @@ class Dataset:
         self.device, self.dtype = dev, dtype
         put = lambda a: torch.from_numpy(np.ascontiguousarray(a[order])).to(dev, dtype)  # noqa: E731
         self.u, self.v = put(u), put(v)
+        # bound on sqrt(u^2 + v^2) of the stored values: the spectrum is built only within it
+        us, vs = (a.astype(np.float32 if dtype == torch.float32 else np.float64).astype(np.float64) for a in (u, v))
+        self.uvmax = float(np.sqrt(np.max(us * us + vs * vs, initial=0.0)))
         self.has_obs = vis_obs_re is not None
         if self.has_obs:
             re, im, w = (to_np(a).astype(np.float64) for a in (vis_obs_re, vis_obs_im, vis_obs_w))
@@ -207,30 +215,33 @@ class Dataset:
         if not self.has_obs:
             raise ValueError('this Dataset has no observed visibilities')
         return chi2_profile(intensity, Rmin, dR, nxy, dxy, self.u, self.v, self.re, self.im, self.w, dRA=dRA,
-                            dDec=dDec, PA=PA, inc=inc, check_central_pixel=check_central_pixel, out=out)
+                            dDec=dDec, PA=PA, inc=inc, check_central_pixel=check_central_pixel, out=out,
+                            uvmax=self.uvmax)
 
     def sample_profile(self, intensity, Rmin, dR, nxy, dxy, dRA=0., dDec=0., PA=0., inc=0.,
                        check_central_pixel=True, out=None):
         """Visibilities in the caller's original point order."""
         return sample_profile(intensity, Rmin, dR, nxy, dxy, self.u, self.v, dRA=dRA, dDec=dDec, PA=PA, inc=inc,
-                              check_central_pixel=check_central_pixel, out=out, perm=self.order)
+                              check_central_pixel=check_central_pixel, out=out, perm=self.order, uvmax=self.uvmax)
 
     def sample_profile_multi(self, intensity, Rmin, dR, nxy, dxy, sources, PA=0., inc=0., check_central_pixel=True,
                              out=None):
         """Multi-source visibilities in the caller's original point order."""
         return sample_profile_multi(intensity, Rmin, dR, nxy, dxy, self.u, self.v, sources, PA=PA, inc=inc,
-                                    check_central_pixel=check_central_pixel, out=out, perm=self.order)
+                                    check_central_pixel=check_central_pixel, out=out, perm=self.order,
+                                    uvmax=self.uvmax)
 
     def chi2_profile_multi(self, intensity, Rmin, dR, nxy, dxy, sources, PA=0., inc=0., check_central_pixel=True,
                            out=None):
         if not self.has_obs:
             raise ValueError('this Dataset has no observed visibilities')
         return chi2_profile_multi(intensity, Rmin, dR, nxy, dxy, self.u, self.v, self.re, self.im, self.w, sources,
-                                  PA=PA, inc=inc, check_central_pixel=check_central_pixel, out=out)
+                                  PA=PA, inc=inc, check_central_pixel=check_central_pixel, out=out,
+                                  uvmax=self.uvmax)
 
 
 def chi2_profile_grad(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_im, vis_obs_w, dRA=0., dDec=0., PA=0.,
-                      inc=0., check_central_pixel=True):
+                      inc=0., check_central_pixel=True, uvmax=0.):
     """chi2 and its gradient on device tensors.
 
     Returns (chi2 [1], grad_intensity [nr], grad_geom [4] = d/d(inc, PA, dRA, dDec)), float64 device tensors."""
@@ -246,7 +257,7 @@ def chi2_profile_grad(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_i
     with torch.cuda.device(dev):
         _check(f(_stream_ptr(dev), intensity.numel(), _p(intensity), Rmin, dR, dxy, int(nxy), inc, dRA, dDec,
                  1.0 / (dxy * nxy), PA, nd, _p(u), _p(v), _p(vis_obs_re), _p(vis_obs_im), _p(vis_obs_w),
-                 int(bool(check_central_pixel)), _p(chi2), _p(gI), _p(gg)))
+                 int(bool(check_central_pixel)), _p(chi2), _p(gI), _p(gg), float(uvmax)))
     return chi2, gI, gg
 
 
@@ -254,7 +265,7 @@ def _dataset_grad(self, intensity, Rmin, dR, nxy, dxy, dRA=0., dDec=0., PA=0., i
     if not self.has_obs:
         raise ValueError('this Dataset has no observed visibilities')
     return chi2_profile_grad(intensity, Rmin, dR, nxy, dxy, self.u, self.v, self.re, self.im, self.w, dRA=dRA,
-                             dDec=dDec, PA=PA, inc=inc, check_central_pixel=check_central_pixel)
+                             dDec=dDec, PA=PA, inc=inc, check_central_pixel=check_central_pixel, uvmax=self.uvmax)
 
 
 Dataset.chi2_profile_grad = _dataset_grad
@@ -333,7 +344,7 @@ class BatchedChi2Profile:
                 raise ValueError('all datasets must share dtype and device')
             self._keep.append(ds)
             arr[k] = St(int(nxy), dxy, 1.0 / (dxy * nxy), ds.n, ds.u.data_ptr(), ds.v.data_ptr(),
-                        ds.re.data_ptr(), ds.im.data_ptr(), ds.w.data_ptr())
+                        ds.re.data_ptr(), ds.im.data_ptr(), ds.w.data_ptr(), ds.uvmax)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             _check(getattr(lib, f'galario_batch_create_{suf}')(len(sets), arr, self.nr, Rmin, dR, self.max_walkers,

@@ -157,7 +157,13 @@ int galario_reduce_chi2_f32(int nd, const float *vis_obs_re, const float *vis_ob
                             const float *weights, float *chi2);
 
 /* ---- device-resident forms (no counterpart in galario: the hot path with
- *      the data already in HBM, for torch callers and the benchmark) ------ */
+ *      the data already in HBM, for torch callers and the benchmark) ------
+ *
+ * uvmax (last argument, wavelengths): a bound on sqrt(u^2 + v^2) over the
+ * points, or <= 0 for none.  Rotation keeps that radius, so with a bound the
+ * spectrum is stored (and its adjoint read) only within uvmax/duv + 3 cells
+ * of the origin; a point beyond the bound is treated as outside the grid
+ * (NaN result, flag set), never read from unstored cells. */
 
 /* galario::chi2_profile (src/galario.cpp:1615) on device arrays; *chi2_d
  * (device, double) receives chi2.  Enqueued on `stream`, not synchronised.
@@ -167,12 +173,12 @@ int galario_chi2_profile_dev_f64(void *stream, int nr, const double *intensity, 
                                  double dxy, int nxy, double inc, double dRA, double dDec, double duv,
                                  double PA, int nd, const double *u, const double *v, const double *vis_obs_re,
                                  const double *vis_obs_im, const double *weights, int check_central,
-                                 double *chi2_d);
+                                 double *chi2_d, double uvmax);
 int galario_chi2_profile_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
                                  float dxy, int nxy, float inc, float dRA, float dDec, float duv,
                                  float PA, int nd, const float *u, const float *v, const float *vis_obs_re,
                                  const float *vis_obs_im, const float *weights, int check_central,
-                                 double *chi2_d);
+                                 double *chi2_d, double uvmax);
 
 /* galario::sample_profile (src/galario.cpp:1404) on device arrays; vis_d: nd
  * complex.  perm (device, may be null): point i is written to vis_d[perm[i]],
@@ -181,11 +187,11 @@ int galario_chi2_profile_dev_f32(void *stream, int nr, const float *intensity, f
 int galario_sample_profile_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
                                    double dxy, int nxy, double inc, double dRA, double dDec, double duv,
                                    double PA, int nd, const double *u, const double *v, int check_central,
-                                   const int *perm, void *vis_d);
+                                   const int *perm, void *vis_d, double uvmax);
 int galario_sample_profile_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
                                    float dxy, int nxy, float inc, float dRA, float dDec, float duv,
                                    float PA, int nd, const float *u, const float *v, int check_central,
-                                   const int *perm, void *vis_d);
+                                   const int *perm, void *vis_d, double uvmax);
 
 /* chi2 of a profile and its gradient (no counterpart in galario): *chi2_d,
  * grad_I[0..nr) = dchi2/dintensity, grad_geom[0..3] = dchi2/d(inc, PA, dRA,
@@ -197,12 +203,14 @@ int galario_chi2_profile_grad_dev_f64(void *stream, int nr, const double *intens
                                       double dxy, int nxy, double inc, double dRA, double dDec, double duv,
                                       double PA, int nd, const double *u, const double *v,
                                       const double *vis_obs_re, const double *vis_obs_im, const double *weights,
-                                      int check_central, double *chi2_d, double *grad_I, double *grad_geom);
+                                      int check_central, double *chi2_d, double *grad_I, double *grad_geom,
+                                      double uvmax);
 int galario_chi2_profile_grad_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
                                       float dxy, int nxy, float inc, float dRA, float dDec, float duv,
                                       float PA, int nd, const float *u, const float *v,
                                       const float *vis_obs_re, const float *vis_obs_im, const float *weights,
-                                      int check_central, double *chi2_d, double *grad_I, double *grad_geom);
+                                      int check_central, double *chi2_d, double *grad_I, double *grad_geom,
+                                      double uvmax);
 
 /* Several sources with the same profile, inclination and PA at their own
  * offsets (galario's multi-source recipe, docs/tech-specs.rst:45-57: sum the
@@ -214,40 +222,44 @@ int galario_chi2_profile_grad_dev_f32(void *stream, int nr, const float *intensi
 int galario_sample_profile_multi_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
                                          double dxy, int nxy, double inc, double duv, double PA, int nsrc,
                                          const double *src, int nd, const double *u, const double *v,
-                                         int check_central, const int *perm, void *vis_d);
+                                         int check_central, const int *perm, void *vis_d, double uvmax);
 int galario_sample_profile_multi_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
                                          float dxy, int nxy, float inc, float duv, float PA, int nsrc,
                                          const float *src, int nd, const float *u, const float *v,
-                                         int check_central, const int *perm, void *vis_d);
+                                         int check_central, const int *perm, void *vis_d, double uvmax);
 int galario_chi2_profile_multi_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
                                        double dxy, int nxy, double inc, double duv, double PA, int nsrc,
                                        const double *src, int nd, const double *u, const double *v,
                                        const double *vis_obs_re, const double *vis_obs_im, const double *weights,
-                                       int check_central, double *chi2_d);
+                                       int check_central, double *chi2_d, double uvmax);
 int galario_chi2_profile_multi_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
                                        float dxy, int nxy, float inc, float duv, float PA, int nsrc,
                                        const float *src, int nd, const float *u, const float *v,
                                        const float *vis_obs_re, const float *vis_obs_im, const float *weights,
-                                       int check_central, double *chi2_d);
+                                       int check_central, double *chi2_d, double uvmax);
 
 /* ---- batched form (no counterpart in galario, whose callers loop over
  *      chi2Profile per walker and dataset, docs/quickstart.rst:147-174) ---- */
 
 /* A dataset of the batched form: device arrays of nd points (the caller keeps
  * them alive and unchanged while a plan uses them), its grid nxy (a power of
- * two from 128 to 8192 in double, 16384 in float) and cell size dxy, and
- * duv (galario passes 1/(nxy*dxy), python/libcommon.pyx:723). */
+ * two from 128 to 8192 in double, 16384 in float) and cell size dxy, duv
+ * (galario passes 1/(nxy*dxy), python/libcommon.pyx:723), and uvmax as for
+ * the device-resident forms (<= 0: none; a grid's spectrum is cut only when
+ * every dataset on it gives one). */
 typedef struct {
     int nxy;
     double dxy, duv;
     int nd;
     const double *u, *v, *vis_obs_re, *vis_obs_im, *weights;
+    double uvmax;
 } galario_dataset_f64;
 typedef struct {
     int nxy;
     float dxy, duv;
     int nd;
     const float *u, *v, *vis_obs_re, *vis_obs_im, *weights;
+    double uvmax;
 } galario_dataset_f32;
 
 /* Plan for up to max_walkers parameter sets, each a profile of nr values on

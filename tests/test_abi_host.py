@@ -164,9 +164,10 @@ def test_gradient_argument_checks_before_any_gpu_work():
     a = galario_amd.arcsec
     f = _lib.lib.galario_chi2_profile_grad_dev_f64
     f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p] + [ctypes.c_double] * 3 + [ctypes.c_int] + \
-        [ctypes.c_double] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 3
+        [ctypes.c_double] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 3 + \
+        [ctypes.c_double]
     p = ctypes.c_void_p(0x1000)
-    common = (0.01 * a, 1024, 0.5, 0.0, 0.0, 1 / (1024 * 0.01 * a), 0.3, 10, p, p, p, p, p, 0, p, p, p)
+    common = (0.01 * a, 1024, 0.5, 0.0, 0.0, 1 / (1024 * 0.01 * a), 0.3, 10, p, p, p, p, p, 0, p, p, p, 0.0)
     with pytest.raises(ValueError, match="up to 2048"):
         _lib.check(f(None, 3000, p, 1e-4 * a, 0.0005 * a, *common))
     with pytest.raises(ValueError, match="not supported by the gradient"):
