@@ -79,6 +79,20 @@ def measured_traffic(kernel, nxy, nvis):
     return None
 
 
+def host_cpu() -> str:
+    """CPU model and logical CPU count of this host (for the CPU baseline)."""
+    model = 'unknown CPU'
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f'{model}, {os.cpu_count()} logical CPUs'
+
+
 def batched_rates(ta, ds, wl, I, chi2_plain, counts, counts_grad, steps, stream):
     """Batched form over the same dataset: P parameter sets (walker 0 = the plain call's parameters, the
     others jittered: profile x lognormal(0.1), inc, PA, dRA, dDec perturbed) per launch of the captured
@@ -444,10 +458,22 @@ def main():
                                   wl.PA, wl.u, wl.v, wl.re, wl.im, wl.w, workers=ncores)
             times.append(time.perf_counter() - t)
         keep = times[1:] if len(times) > 1 else times
+        # the same path on one thread, over a few evaluations (about 1 s each)
+        port.threads(1)
+        t1 = []
+        tstart = time.perf_counter()
+        while time.perf_counter() - tstart < min(4.0, args.cpu_seconds) and len(t1) < 4:
+            t = time.perf_counter()
+            port.chi2_profile(wl.intensity, wl.Rmin, wl.dR, wl.dxy, wl.nxy, wl.inc, wl.dRA, wl.dDec, wl.duv,
+                              wl.PA, wl.u, wl.v, wl.re, wl.im, wl.w, workers=1)
+            t1.append(time.perf_counter() - t)
+        port.threads(ncores)
         cpu = {'value': round(1.0 / float(np.mean(keep)), 4), 'unit': 'evals/s', 'cores': ncores, 'kind': 'port',
                'sample': f'{len(times)} full-size evaluations (first dropped), C/OpenMP restatement of galario '
                          f'CPU loops + scipy.fft.rfft2(workers={ncores}); chi2 rel. diff vs GPU '
-                         f'{abs(c - chi2_host) / abs(chi2_host):.1e}'}
+                         f'{abs(c - chi2_host) / abs(chi2_host):.1e}',
+               'one_thread_evals_per_s': round(len(t1) / float(np.sum(t1)), 4) if t1 else None,
+               'host': host_cpu()}
 
     res = {
         'metric': METRIC, 'value': round(evals, 2), 'unit': 'evals/s', 'n_gpus': 1, 'steps': nsteps,
