@@ -59,30 +59,37 @@ __device__ __forceinline__ T sweep_pixel(int xi, int yi, const SweepPrm<T>& p) {
     return p.sr_to_px * (I0 + (r - T(iR) * p.dR - p.Rmin) * (I1 - I0) / p.dR);
 }
 
-template <typename T>
-__device__ __noinline__ T central_value(const SweepPrm<T>& p) {
-    const T* I = p.I;
+// Central pixel (create_image_d's central value, galario.cpp:1135-1154): the
+// profile's flux inside the pixel's inscribed disc over its area.  I(i)
+// returns profile value i; it is read for i = 0..iIN+1.
+template <typename T, class Prof>
+__device__ __forceinline__ T central_value_of(const SweepPrm<T>& p, Prof I) {
     const int iIN = p.iIN;
     const T Rmin = p.Rmin, dR = p.dR;
     T flux = T(0);
-    for (int i = 1; i < iIN; ++i) flux += (Rmin + dR * T(i)) * I[i];
+    for (int i = 1; i < iIN; ++i) flux += (Rmin + dR * T(i)) * I(i);
     flux = T(double(flux) * 2.0);
-    flux += Rmin * I[0] + (Rmin + T(iIN) * dR) * I[iIN];
+    flux += Rmin * I(0) + (Rmin + T(iIN) * dR) * I(iIN);
     flux *= dR;
     const double half = double(p.dxy) / 2.0;
     if constexpr (std::is_same<T, double>::value) {
-        const double I_interp = (I[iIN + 1] - I[iIN]) / dR * (half - (Rmin + dR * iIN)) + I[iIN];
-        flux += ((Rmin + iIN * dR) * I[iIN] + half * I_interp) * (half - (Rmin + iIN * dR));
+        const double I_interp = (I(iIN + 1) - I(iIN)) / dR * (half - (Rmin + dR * iIN)) + I(iIN);
+        flux += ((Rmin + iIN * dR) * I(iIN) + half * I_interp) * (half - (Rmin + iIN * dR));
         const double area = half * half - Rmin * Rmin;
         return p.sr_to_px * flux / area;
     } else {
         const float edge = Rmin + dR * float(iIN);
-        const float I_interp = float(double((I[iIN + 1] - I[iIN]) / dR) * (half - double(edge)) + double(I[iIN]));
-        const double term = (double(edge * I[iIN]) + half * double(I_interp)) * (half - double(edge));
+        const float I_interp = float(double((I(iIN + 1) - I(iIN)) / dR) * (half - double(edge)) + double(I(iIN)));
+        const double term = (double(edge * I(iIN)) + half * double(I_interp)) * (half - double(edge));
         flux = float(double(flux) + term);
         const float area = float(half * half - double(Rmin) * double(Rmin));
         return p.sr_to_px * flux / area;
     }
+}
+
+template <typename T>
+__device__ __noinline__ T central_value(const SweepPrm<T>& p) {
+    return central_value_of(p, [&](int i) { return p.I[i]; });
 }
 
 // Value of quadrant-shifted image pixel (row k, column n) built from a profile.
@@ -329,28 +336,42 @@ __device__ __forceinline__ void rowpass_sym_body(const SymRowArgs<T>& a, int blk
     constexpr int N = S::N, NT = S::NT, E = S::E;
     constexpr int C = N / 2 + 1;
     __shared__ T lds[F::LDS_REALS];
+    __shared__ T Ic[NT];
     const int tid = threadIdx.x;
     const int ma = 2 * blk, mb = ma + 1;
     const bool hb = mb < a.nrows;
+    const bool cen = ma == 0;     // this workgroup's row ma = 0 holds the central pixel
     if (blk == 0) {
         if (a.zero)
             for (int k = tid; k < a.nzero; k += NT) a.zero[k] = 0.0;
         if (a.zero1 && tid == 0) *a.zero1 = 0.0;
     }
+    // The central pixel is row 0's input n = 0: a constant that adds its value
+    // to every output frequency.  The transform runs without it and the value
+    // is added at the store, so its profile reads (issued here, beside the
+    // sweep's) and its serial arithmetic stay off the transform's path.
+    T ipre = T(0);
+    if (cen && tid < a.sw.nr) ipre = a.sw.I[tid];
     cpx<T> v[E];
     F::run(v, lds, tid, a.Rs, a.tw, [&](int n) -> cpx<T> {
         const int xi = n < N / 2 ? -n : N - n;
-        const T za = (ma == 0 && n == 0) ? central_value(a.sw) : sweep_pixel(xi, ma, a.sw);
+        const T za = (ma == 0 && n == 0) ? T(0) : sweep_pixel(xi, ma, a.sw);
         const T zb = hb ? sweep_pixel(xi, mb, a.sw) : T(0);
         return cpx<T>{za, zb};
     });
+    T c = T(0);
+    if (cen) {
+        Ic[tid] = ipre;
+        __syncthreads();
+        c = central_value_of(a.sw, [&](int i) { return i < NT ? Ic[i] : a.sw.I[i]; });
+    }
     // rows ma, mb of Ts (mb = nrows is a padding row): four consecutive lanes fill one line
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         const int k = tid + r * NT;
         if (r * NT < C && k < C) {
             T* o = a.Ts + ts_index(ma, k, a.ldT);
-            o[0] = v[r].re;
+            o[0] = cen ? v[r].re + c : v[r].re;
             o[4] = v[r].im;
         }
     }
