@@ -423,18 +423,22 @@ __device__ __forceinline__ double sweep_pixel_adjoint(int xi, int yi, double gz,
     return gz * dvdr * xs2 * tan_inc / double(r);
 }
 
+// Bins i = t, t + nt, ... of the central pixel's adjoint (all threads t < nt
+// of a workgroup together cover bins 0..iIN+1).
 template <typename T>
-__device__ __noinline__ void central_adjoint(double gz, const SweepPrm<T>& p, double* gI) {
+__device__ __forceinline__ void central_adjoint(double gz, const SweepPrm<T>& p, double* gI, int t, int nt) {
     const int iIN = p.iIN;
     const double Rmin = p.Rmin, dR = p.dR, h = double(p.dxy) / 2.0;
     const double area = h * h - Rmin * Rmin;
     const double sc = gz * double(p.sr_to_px) / area;
     const double RiIN = Rmin + iIN * dR;
     const double delta = h - RiIN, beta = delta / dR;
-    lds_add_f64(gI + 0, sc * dR * Rmin);
-    for (int i = 1; i < iIN; ++i) lds_add_f64(gI + i, sc * 2.0 * dR * (Rmin + i * dR));
-    lds_add_f64(gI + iIN, sc * (dR * RiIN + delta * (RiIN + h * (1.0 - beta))));
-    lds_add_f64(gI + iIN + 1, sc * delta * h * beta);
+    for (int i = t; i <= iIN + 1; i += nt) {
+        if (i == 0) lds_add_f64(gI + 0, sc * dR * Rmin);
+        if (i >= 1 && i < iIN) lds_add_f64(gI + i, sc * 2.0 * dR * (Rmin + i * dR));
+        if (i == iIN) lds_add_f64(gI + iIN, sc * (dR * RiIN + delta * (RiIN + h * (1.0 - beta))));
+        if (i == iIN + 1) lds_add_f64(gI + iIN + 1, sc * delta * h * beta);
+    }
 }
 
 constexpr int ADJROW_LDS_NR = 2048;   // profile bins accumulated in LDS by the adjoint row pass
@@ -460,6 +464,7 @@ __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk)
         const T wk = T(ext_weight(k, N));
         return cpx<T>{wk * ra[l], hb ? wk * rb[l] : T(0)};
     }, a.Rs < NT);        // pixels j <= Rs' only
+    __shared__ double gcen;
     double ginc = 0.0;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
@@ -467,10 +472,14 @@ __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk)
         if (r * NT <= a.Rs && j <= a.Rs) {
             const double cj = mult(j, N);
             const double gza = cj * double(v[r].re), gzb = cj * double(v[r].im);
-            if (ma == 0 && j == 0) central_adjoint(gza, a.sw, gIl);
+            if (ma == 0 && j == 0) gcen = gza;      // the central pixel: spread over the workgroup below
             else ginc += sweep_pixel_adjoint(j, ma, gza, a.sw, a.tan_inc, gIl);
             if (hb) ginc += sweep_pixel_adjoint(j, mb, gzb, a.sw, a.tan_inc, gIl);
         }
+    }
+    if (ma == 0) {                                 // (uniform per workgroup)
+        __syncthreads();
+        central_adjoint(gcen, a.sw, gIl, tid, NT);
     }
     const double b = block_sum_any(ginc, red);    // (its barrier also orders the LDS sums before the flush)
     if (tid == 0 && b != 0.0) atomic_add_f64(a.ginc, b);
