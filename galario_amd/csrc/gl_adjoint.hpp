@@ -279,8 +279,7 @@ __device__ __forceinline__ void sum_partials_in_block(const double* __restrict__
     double acc[V];
 #pragma unroll
     for (int q = 0; q < V; ++q) {
-        acc[q] = 0.0;
-        for (int i = t + q * NT; i < n; i += 1024) acc[q] += p[i];
+        acc[q] = sum_stride1024(p, t + q * NT, n, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) {

@@ -215,8 +215,7 @@ __global__ __launch_bounds__(1024) void k_batch_reduce(const double* __restrict_
     for (int s = 0; s < sets.n; ++s) {
         for (int k = 0; k < nstream; ++k) {
             const double* p = part + (size_t(w) * 4 + k) * ldPart + sets.off[s];
-            double acc = 0.0;
-            for (int i = threadIdx.x; i < sets.nb[s]; i += 1024) acc += p[i];
+            double acc = sum_stride1024(p, threadIdx.x, sets.nb[s], 0.0);
             acc = wave_sum(acc);
             if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
             __syncthreads();

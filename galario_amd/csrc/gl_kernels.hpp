@@ -603,6 +603,20 @@ __device__ __forceinline__ double block_sum256(double acc, double* red4) {
     return (red4[0] + red4[1]) + (red4[2] + red4[3]);
 }
 
+// acc + p[i0] + p[i0 + 1024] + ... (terms below n), added in that order; the
+// loads of eight terms are issued together, one memory round trip per eight.
+__device__ __forceinline__ double sum_stride1024(const double* __restrict__ p, int i0, int n, double acc) {
+    for (; i0 < n; i0 += 8 * 1024) {
+        double x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = i0 + q * 1024 < n ? p[i0 + q * 1024] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (i0 + q * 1024 < n) acc += x[q];
+    }
+    return acc;
+}
+
 // Deterministic final sum of n per-workgroup partials (one workgroup of 1024
 // threads, fixed order).  A separate launch: publishing partials across the
 // 8 XCDs from inside one kernel would need a device-scope release per
@@ -610,8 +624,7 @@ __device__ __forceinline__ double block_sum256(double acc, double* red4) {
 template <typename T>
 __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict__ p, int n, double* out_d, T* out_t) {
     __shared__ double red[16];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) acc += p[i];
+    double acc = sum_stride1024(p, threadIdx.x, n, 0.0);
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
