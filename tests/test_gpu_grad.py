@@ -158,6 +158,28 @@ def test_gradient_vs_finite_differences():
     assert not bad, bad
 
 
+def test_gradient_with_many_inner_bins():
+    """dR = dxy/300 puts ~150 profile bins inside the central pixel, more than
+    the 64 threads of the N=256 row passes: the forward central value reads
+    bins beyond them from memory and the central pixel's adjoint is spread
+    over the workgroup in several rounds.  Against the torch reference, as in
+    test_gradient_vs_torch_reference; bins 0..iIN+1 get their gradient from
+    the central pixel alone."""
+    N, dxy = 256, 0.01 * arcsec
+    dR = dxy / 300
+    d = problem(N=N, n=4000, dxy=dxy, dR=dR, nr=1500)
+    iIN = math.floor((dxy / 2 - d['Rmin']) / dR)
+    assert iIN > 64
+    c, gI, gg = gpu_grad(d)
+    c_ref, gI_ref, gg_ref = torch_ref_grad(d)
+    assert math.isclose(c, c_ref, rel_tol=1e-11)
+    assert np.linalg.norm(gI - gI_ref) <= 1e-7 * np.linalg.norm(gI_ref)
+    inner = slice(0, iIN + 2)
+    assert np.linalg.norm(gI[inner] - gI_ref[inner]) <= 1e-7 * np.linalg.norm(gI_ref[inner])
+    for k in range(4):
+        assert math.isclose(gg[k], gg_ref[k], rel_tol=1e-8), (k, gg[k], gg_ref[k])
+
+
 def test_repeat_calls_and_forward_consistency():
     from galario_amd import torch_api as ta
     d = problem(N=256, n=4000)

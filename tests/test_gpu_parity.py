@@ -344,6 +344,32 @@ def test_generic_sizes_sample_image(N):
     assert_vis_close(got, ref, 1e-12)
 
 
+@pytest.mark.parametrize("nxy", [512, 4096])
+def test_central_pixel_with_many_inner_bins(nxy):
+    """A fine profile (dR = dxy/700) puts ~350 profile bins inside the central
+    pixel, more than the row pass has threads (64 at nxy=512, 256 at 4096):
+    the central value then reads some bins from LDS and the rest from the
+    profile in memory.  sampleProfile and chi2Profile against the oracle at
+    1e-12, as in test_all_double."""
+    nsamples = 2000
+    u, v = uv_points(nsamples, 3e3)
+    dxy = 1. / (np.hypot(u, v).max() * 4.8)
+    Rmin, dR, nrad, inc = dxy / 1000., dxy / 700., 3000, 30. * deg
+    assert math.floor((dxy / 2 - Rmin) / dR) > 256
+    dRA, dDec, PA = 1.1 * arcsec, -0.7 * arcsec, 35. * deg
+    I = radial_profile(Rmin, dR, nrad, 'Gauss', dxy * 2)
+    duv = 1 / (dxy * nxy)
+    img = O.sweep(I, Rmin, dR, nxy, dxy, inc)
+    ref = O.sample_image(img, 1.0, dRA, dDec, duv, PA, u, v)
+    vis = gD.sampleProfile(I, Rmin, dR, nxy, dxy, u, v, dRA=dRA, dDec=dDec, PA=PA, inc=inc)
+    assert_vis_close(vis, ref, 1e-12)
+    rng = np.random.default_rng(6)
+    re, im = ref.real + rng.standard_normal(nsamples), ref.imag + rng.standard_normal(nsamples)
+    w = rng.uniform(0.5, 2.0, nsamples)
+    c = gD.chi2Profile(I, Rmin, dR, nxy, dxy, u, v, re, im, w, dRA=dRA, dDec=dDec, PA=PA, inc=inc)
+    assert math.isclose(c, O.reduce_chi2(re, im, ref, w), rel_tol=1e-12)
+
+
 def test_profile_larger_than_image():
     """rmax capped at nxy/2: the swept box is the whole image."""
     Rmin, dR, nr, N, dxy = 1e-4 * arcsec, 0.002 * arcsec, 3000, 256, 0.01 * arcsec
