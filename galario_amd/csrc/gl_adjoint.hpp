@@ -51,7 +51,7 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtom
 // ---------------------------------------------------------------------------
 constexpr int GHOT = 16;          // hot tile: columns [0, GHOT) x folded rows [0, GHOT)
 constexpr int GREP = 32;          // replicas of the hot tile in global memory
-constexpr int GWIN = 4096;        // LDS window capacity (cells)
+constexpr int GWIN = 3072;        // LDS window capacity (cells): 26 KB with the hot tile, 6 waves per SIMD
 constexpr int GHOT_ELEMS = GREP * GHOT * GHOT;
 
 template <typename T>

@@ -454,8 +454,14 @@ __device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk
     colpass_nyquist<T, LOG2N>(a, pair);
 }
 
+// At most 128 VGPRs (4 waves per SIMD): one call's N/4 workgroups then all fit
+// on the chip at once (at 130 VGPRs a quarter of them waited for a second
+// round); the cap costs a 20-byte spill at N=4096 FP64 and still saves ~6%.
+// The batched kernel, whose many walkers fill the chip regardless, keeps the
+// compiler's choice.
 template <typename T, int LOG2N>
-__global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass_sym(SymColArgs<T> a) {
+__global__ __launch_bounds__(FftShape<LOG2N>::NT) __attribute__((amdgpu_waves_per_eu(4))) void k_colpass_sym(
+    SymColArgs<T> a) {
     colpass_sym_body<T, LOG2N>(a, blockIdx.x, gridDim.x);
 }
 
