@@ -438,6 +438,29 @@ def main():
 
     extras = extra_configs(ta, g, stream, args.steps) if args.extras else None
 
+    def read_rate_gbps(nbytes, grids=(1024, 2048, 4096), reps=20):
+        """Best rate of a plain 16-byte-load stream over `nbytes` of device memory (the library's
+        read-rate reference kernel), `reps` back-to-back launches per grid between two events."""
+        buf = torch.zeros(nbytes // 16 * 2, dtype=torch.float64, device=dev)
+        sums = torch.empty(max(grids), dtype=torch.float64, device=dev)
+        best = 0.0
+        for gr in grids:
+            for _ in range(3):
+                ta.read_probe(buf, gr, sums)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                ta.read_probe(buf, gr, sums)
+            e1.record(stream)
+            e1.synchronize()
+            best = max(best, buf.numel() * 8 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+        del buf, sums
+        return best
+
+    # plain streams of the sample kernel's bytes and of 1 GiB: what reads of that size reach here
+    read_same = read_rate_gbps(sb['sample'])
+    read_large = read_rate_gbps(1 << 30, reps=5)
+
     # host-pointer (drop-in) form, PCIe uploads included: a few calls
     nh = 5
     t = time.perf_counter()
@@ -487,7 +510,10 @@ def main():
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
                      'alg_bytes_per_launch': kernels[dom]['alg_bytes'],
-                     'avg_launch_ms': kernels[dom]['avg_launch_ms']},
+                     'avg_launch_ms': kernels[dom]['avg_launch_ms'],
+                     'stream_read_same_bytes_GBps': round(read_same, 1),
+                     'stream_read_1GiB_GBps': round(read_large, 1),
+                     'achieved_vs_stream_read_same_bytes': round(achieved / read_same, 4)},
         'kernels': kernels,
         'pipeline_roofline': {'alg_bytes_per_eval': b_alg,
                               'frac': round(b_alg * evals / (HBM_PEAK_GBS * 1e9), 4),

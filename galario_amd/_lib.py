@@ -73,6 +73,7 @@ def _load():
     lib.galario_free.argtypes = [_vp]
     lib.galario_free.restype = None
     lib.galario_take_oob_flag.argtypes = [_ip]
+    lib.galario_read_probe.argtypes = [_vp, _vp, ctypes.c_longlong, _c_int, _vp]
     lib.galario_stage_timer.argtypes = [_c_int]
     lib.galario_stage_repeat.argtypes = [_c_int, _c_int]
     lib.galario_stage_times.argtypes = [ctypes.POINTER(ctypes.c_double), _c_int, _ip]

@@ -1763,6 +1763,18 @@ int galario_batch_take_oob_flag(void* plan, int* out) {
     });
 }
 
+int galario_read_probe(void* stream, const void* buf, long long bytes, int grid, double* out) {
+    return guard([&] {
+        if (!buf || !out) invalid("read_probe", "null buffer");
+        if (bytes < 0 || bytes % 16 != 0) invalid("read_probe", "bytes must be a non-negative multiple of 16");
+        if (grid < 1 || grid > 65536) invalid("read_probe", "grid must be in [1, 65536]");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(k_read_probe, dim3(grid), dim3(256), 0, s, static_cast<const cpx<double>*>(buf),
+                           size_t(bytes / 16), out);
+        HC(hipGetLastError());
+    });
+}
+
 int galario_stage_timer(int enable) {
     return guard([&] {
         t_timer.on = enable != 0;

@@ -601,6 +601,11 @@ __device__ __forceinline__ double block_sum_any(double acc, double* red16) {
     return s;
 }
 
+// Read-rate reference: the workgroups read n2 16-byte elements in a
+// grid-stride loop and write one sum each.  A plain stream of the same bytes
+// as a kernel bounds the rate that kernel's own reads can reach.
+__global__ __launch_bounds__(256) void k_read_probe(const cpx<double>* __restrict__ p, size_t n2, double* out);
+
 // Deterministic block sum of `acc` (blockDim 256) into *out on thread 0.
 __device__ __forceinline__ double block_sum256(double acc, double* red4) {
     acc = wave_sum(acc);
@@ -621,6 +626,18 @@ __device__ __forceinline__ double sum_stride1024(const double* __restrict__ p, i
             if (i0 + q * 1024 < n) acc += x[q];
     }
     return acc;
+}
+
+__global__ __launch_bounds__(256) void k_read_probe(const cpx<double>* __restrict__ p, size_t n2, double* out) {
+    __shared__ double red[16];
+    double s = 0.0;
+    const size_t stride = size_t(gridDim.x) * 256;
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n2; i += stride) {
+        const cpx<double> x = p[i];
+        s += x.re + x.im;
+    }
+    s = block_sum_any(s, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 // Deterministic final sum of n per-workgroup partials (one workgroup of 1024

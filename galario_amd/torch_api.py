@@ -112,6 +112,18 @@ def stage_repeat(stage, reps: int) -> None:
     _check(lib.galario_stage_repeat(idx, int(reps)))
 
 
+def read_probe(buf: torch.Tensor, grid: int, out: torch.Tensor) -> None:
+    """HBM read-rate reference: `grid` workgroups read all of the contiguous device
+    tensor `buf` (a multiple of 16 bytes) and write one sum each to `out`
+    (float64, at least `grid` elements), on the current stream."""
+    if buf.device.type != 'cuda' or out.device != buf.device or out.dtype != torch.float64:
+        raise ValueError('read_probe expects a device buffer and a float64 output on the same device')
+    if not (buf.is_contiguous() and out.is_contiguous()) or out.numel() < grid:
+        raise ValueError('read_probe expects contiguous tensors and at least `grid` outputs')
+    nbytes = buf.numel() * buf.element_size()
+    _check(lib.galario_read_probe(_stream_ptr(buf.device), _p(buf), nbytes, int(grid), _p(out)))
+
+
 def stage_times():
     """(summed milliseconds per stage as a dict, number of timed calls)."""
     ms = (ctypes.c_double * len(STAGES))()

@@ -330,6 +330,14 @@ int galario_stage_repeat(int stage, int reps);
  * stage to ms[0..nstage-1] and the number of timed calls to *ncalls. */
 int galario_stage_times(double *ms, int nstage, int *ncalls);
 
+/* Read-rate reference for the measurement tools (no counterpart in galario):
+ * `grid` workgroups of 256 threads read `bytes` (a multiple of 16) of device
+ * memory `buf` in 16-byte loads, grid-stride, and write one sum per
+ * workgroup to the device array out[0..grid-1], on `stream`.  Timing it over
+ * the bytes a kernel reads gives the rate a plain stream of that size
+ * reaches on this GPU. */
+int galario_read_probe(void *stream, const void *buf, long long bytes, int grid, double *out);
+
 /* Synchronises the calling thread's work on the current device and returns
  * (and clears) its out-of-grid flag: 1 if any point of a _dev call since the
  * last take fell outside the Fourier grid. */

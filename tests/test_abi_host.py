@@ -172,3 +172,17 @@ def test_gradient_argument_checks_before_any_gpu_work():
         _lib.check(f(None, 3000, p, 1e-4 * a, 0.0005 * a, *common))
     with pytest.raises(ValueError, match="not supported by the gradient"):
         _lib.check(f(None, 1000, p, 1e-4 * a, 0.0005 * a, 0.01 * a, 1000, *common[2:]))
+
+
+def test_read_probe_argument_checks_before_any_gpu_work():
+    """The read-rate reference rejects bad sizes and null buffers on the host."""
+    f = _lib.lib.galario_read_probe
+    p = ctypes.c_void_p(0x1000)
+    with pytest.raises(ValueError, match="null buffer"):
+        _lib.check(f(None, None, 16, 1, p))
+    with pytest.raises(ValueError, match="multiple of 16"):
+        _lib.check(f(None, p, 24, 1, p))
+    with pytest.raises(ValueError, match="grid must be"):
+        _lib.check(f(None, p, 32, 0, p))
+    with pytest.raises(ValueError, match="grid must be"):
+        _lib.check(f(None, p, 32, 70000, p))

@@ -118,3 +118,18 @@ def test_stage_repeat_leaves_results_unchanged():
         ta.stage_timer(False)
     with pytest.raises(ValueError):
         ta.stage_repeat('adjoint_row_pass', 2)
+
+
+@pytest.mark.parametrize("grid", [1, 7, 1024])
+def test_read_probe_reads_every_element(grid):
+    """The read-rate reference visits each 16-byte element once: its per-workgroup
+    sums add up to the buffer's sum (small integers: exact in FP64)."""
+    from galario_amd import torch_api as ta
+    dev = torch.device('cuda:0')
+    n = 2 * 100_003
+    buf = torch.arange(n, dtype=torch.float64, device=dev) % 1000
+    out = torch.full((grid,), float('nan'), dtype=torch.float64, device=dev)
+    ta.read_probe(buf, grid, out)
+    assert out.sum().item() == buf.sum().item()
+    with pytest.raises(ValueError):
+        ta.read_probe(buf, grid + 1, out)
