@@ -82,7 +82,7 @@ def torch_ref_grad(d):
     c = TR.chi2_profile(I, g, d['Rmin'], d['dR'], d['N'], d['dxy'], T(d['u']), T(d['v']), T(d['re']), T(d['im']),
                         T(d['w']))
     c.backward()
-    return float(c), I.grad.numpy(), g.grad.numpy()
+    return float(c.detach()), I.grad.numpy(), g.grad.numpy()
 
 
 @pytest.mark.parametrize("N, n", [(128, 1500), (256, 4000), (1024, 20000)])
@@ -178,6 +178,29 @@ def test_gradient_with_many_inner_bins():
     assert np.linalg.norm(gI[inner] - gI_ref[inner]) <= 1e-7 * np.linalg.norm(gI_ref[inner])
     for k in range(4):
         assert math.isclose(gg[k], gg_ref[k], rel_tol=1e-8), (k, gg[k], gg_ref[k])
+
+
+@pytest.mark.parametrize("geom", [(0.0, 0.0, 0.0, 0.0), (0.4, 0.0, 0.0, 0.0), (0.0, 0.7, 0.0, 0.0),
+                                  (0.5, 0.0, 0.02, -0.01)],
+                         ids=['all_zero', 'inc_only', 'PA_only', 'PA_zero_with_offset'])
+def test_gradient_at_special_geometry(geom):
+    """PA = 0 skips the rotation and dRA = dDec = 0 the phase factor in the
+    kernels; the gradients there (including d/dPA, d/ddRA, d/ddDec at 0) against
+    the torch reference, as in test_gradient_vs_torch_reference.  Offsets in
+    arcsec."""
+    d = problem(N=256, n=4000, dxy=0.01 * arcsec, dR=0.002 * arcsec, nr=400)
+    g = np.array(geom, dtype=float)
+    g[2:] *= arcsec
+    d['geom'] = g
+    c, gI, gg = gpu_grad(d)
+    c_ref, gI_ref, gg_ref = torch_ref_grad(d)
+    assert math.isclose(c, c_ref, rel_tol=1e-11)
+    assert np.linalg.norm(gI - gI_ref) <= 1e-7 * np.linalg.norm(gI_ref)
+    for k in range(4):
+        if gg_ref[k] == 0.0:          # d/dinc at inc = 0: tan(inc) = 0 in every term
+            assert gg[k] == 0.0, (k, gg[k])
+        else:
+            assert math.isclose(gg[k], gg_ref[k], rel_tol=1e-8), (k, gg[k], gg_ref[k])
 
 
 def test_repeat_calls_and_forward_consistency():
