@@ -270,6 +270,56 @@ def extra_configs(ta, g, stream, steps):
         plan.close()
         del sets, plan
     out['batched_64_sets_3_datasets_nxy4096_grad'] = res
+
+    # multi-source summation: K sources sharing the profile, inc and PA, on the main workload's shape
+    wl = syn.ring_profile_workload(model, nvis=1_000_000, nxy=4096, seed=1)
+    ds = ta.Dataset(wl.u, wl.v, wl.re, wl.im, wl.w, device=dev, dtype=f64)
+    It = torch.from_numpy(np.ascontiguousarray(wl.intensity)).to(dev)
+    args = (It, wl.Rmin, wl.dR, wl.nxy, wl.dxy)
+    res = {'workload': 'chi2 of K sources (flux, dRA, dDec) sharing one profile, inc and PA, nxy=4096, '
+                       'Nvis=1e6, FP64 (one spectrum, K phase-shifted samples per point)'}
+    for K in (1, 4):
+        src = torch.tensor([[1.0 / K, wl.dRA * (1 + 0.5 * k), wl.dDec * (1 - 0.3 * k)] for k in range(K)],
+                           dtype=f64, device=dev)
+        ms = timeit(lambda: ds.chi2_profile_multi(*args, src, PA=wl.PA, inc=wl.inc, check_central_pixel=False),
+                    max(20, min(steps, 200)))
+        res[f'K{K}'] = {'evals_per_s': round(1000.0 / ms, 1), 'ms_per_eval': round(ms, 4)}
+    one = torch.tensor([[1.0, wl.dRA, wl.dDec]], dtype=f64, device=dev)
+    c1 = ds.chi2_profile_multi(*args, one, PA=wl.PA, inc=wl.inc, check_central_pixel=False).item()
+    c0 = ds.chi2_profile(*args, PA=wl.PA, inc=wl.inc, dRA=wl.dRA, dDec=wl.dDec, check_central_pixel=False).item()
+    res['one_source_rel_diff_vs_plain'] = abs(c1 - c0) / abs(c0)
+    out['multisource_nxy4096_nvis1e6'] = res
+    del ds
+
+    # gradient sampler: galario's quickstart posterior (6 parameters) for 64 chains, leapfrog HMC
+    from galario_amd import sampler as S
+    rng = np.random.default_rng(6)
+    nxy, dxy, nR = 1024, 0.02 * A, 500
+    Rmin, dR = 1e-4 * A, 0.01 * A
+    truth = np.array([10.0, 0.4, 50.0, 30.0, 0.05, -0.05])
+    qs = syn.ProfileWorkload(intensity=10 ** truth[0] * np.exp(-0.5 * ((Rmin + dR * np.arange(nR)) /
+                                                                      (truth[1] * A)) ** 2),
+                             Rmin=Rmin, dR=dR, nxy=nxy, dxy=dxy, inc=truth[2] * syn.DEG, PA=truth[3] * syn.DEG,
+                             dRA=truth[4] * A, dDec=truth[5] * A, u=None, v=None)
+    qs.u, qs.v = syn.log_uniform_uv(rng, 200_000, 2 / (nxy * dxy), 0.4 / dxy)
+    re, im, w = syn.noisy_observation(rng, model(qs))
+    ds = ta.Dataset(qs.u, qs.v, re, im, w, device=dev, dtype=f64)
+    W, n_steps, n_leap = 64, 3, 10
+    plan = ta.BatchedChi2Profile([(ds, nxy, dxy)], nR, Rmin, dR, max_walkers=W, check_central_pixel=False)
+    post = S.QuickstartPosterior(plan, [[1, 20], [0, 8], [0, 90], [0, 180], [-2, 2], [-2, 2]], Rmin, dR, nR)
+    x0 = torch.tensor(truth[None, :] + 0.01 * rng.standard_normal((W, 6)) * np.array([1, 1, 10, 10, 1, 1]),
+                      device=dev)
+    scale = torch.tensor([1e-3, 1e-3, 1e-1, 1e-1, 1e-3, 1e-3], dtype=f64, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    S.hmc(post, x0, n_steps=1, step_size=scale, n_leapfrog=2, generator=gen)       # plan set-up and autotune
+    ms = timeit(lambda: S.hmc(post, x0, n_steps=n_steps, step_size=scale, n_leapfrog=n_leap, generator=gen), 2)
+    grads = n_steps * n_leap + 1
+    out['hmc_quickstart_posterior_64_chains'] = {
+        'workload': "galario's quickstart posterior (Gaussian profile, 6 parameters) on 2e5 visibilities, "
+                    f'nxy={nxy}, Nr={nR}, FP64: {W} HMC chains x {n_steps} steps x {n_leap} leapfrog steps',
+        'chain_gradient_evals_per_s': round(W * grads * 1000.0 / ms, 1), 'ms_per_hmc_call': round(ms, 3)}
+    plan.close()
     return out
 
 
