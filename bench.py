@@ -157,7 +157,8 @@ def extra_configs(ta, g, stream, steps):
     """The other configurations of the benchmark suite (each timed like the main one, fewer steps):
     differentiation at nxy=2048 (plain gradient, FP64 and FP32), the batched forward of 256 walkers over
     three datasets of uneven nxy and Nvis, and the batched forward+gradient of 64 sets over three nxy=4096
-    datasets (FP64 and FP32, errors against the FP64 plain path), plus the FP32 plain forward."""
+    datasets (FP64 and FP32, errors against the FP64 plain path), the multi-source chi2 (K = 1 and 4 sources)
+    on the main shape, and vectorised HMC on galario's quickstart posterior (64 chains)."""
     import numpy as np
     import torch
     from galario_amd import synthetic as syn
