@@ -988,7 +988,9 @@ int profile_multisrc_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, 
             sa.w = wt;
             sa.partials = w.part.get<double>(nb ? nb : 1);
             t_timer.mark(ST_SAMPLE, s);
-            if (nb) hipLaunchKernelGGL((k_sample_multisrc<T, MODE_CHI2>), dim3(nb), dim3(256), 0, s, sa, nsrc, src);
+            if (nb)
+                hipLaunchKernelGGL((k_sample_multisrc<T, MODE_CHI2, SAMPLE_PPT>), dim3((nb + SAMPLE_PPT - 1) / SAMPLE_PPT),
+                                   dim3(256), 0, s, sa, nsrc, src, int(nb));
             t_timer.mark(ST_REDUCE, s);
             hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, sa.partials, int(nb), chi2_d,
                                static_cast<T*>(nullptr));
@@ -996,7 +998,9 @@ int profile_multisrc_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, 
             sa.vis = static_cast<cpx<T>*>(vis_d);
             sa.perm = perm;
             t_timer.mark(ST_SAMPLE, s);
-            if (nb) hipLaunchKernelGGL((k_sample_multisrc<T, MODE_VIS>), dim3(nb), dim3(256), 0, s, sa, nsrc, src);
+            if (nb)
+                hipLaunchKernelGGL((k_sample_multisrc<T, MODE_VIS, SAMPLE_PPT>), dim3((nb + SAMPLE_PPT - 1) / SAMPLE_PPT),
+                                   dim3(256), 0, s, sa, nsrc, src, int(nb));
         }
         HC(hipGetLastError());
         t_timer.end_call(s);

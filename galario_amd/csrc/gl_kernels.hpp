@@ -695,55 +695,80 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs<T> a) {
 // share the spectrum and the interpolated value V0 at each point, so
 // V = sum_k flux_k * apply_phase(V0, dRA_k, dDec_k), k in order.
 // src: nsrc x 3 (flux, dRA, dDec) in the caller's (unrotated) frame.
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_sample_multisrc(SampleArgs<T> a, int nsrc, const T* __restrict__ src) {
-    __shared__ double red[4];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    double acc = 0.0;
-    bool oob = false;
-    if (i < a.nd) {
-        SampleArgs<T> b = a;
-        b.phase = 0;
-        const cpx<T> V0 = a.H ? sample_point_sym(b, a.u[i], a.v[i], oob) : sample_point(b, a.u[i], a.v[i], oob);
-        T ur = a.u[i], vr = a.v[i];
+template <typename T>
+__device__ __forceinline__ cpx<T> multisrc_point(const SampleArgs<T>& a, int i, int nsrc, const T* __restrict__ src,
+                                                 bool& oob) {
+    SampleArgs<T> b = a;
+    b.phase = 0;
+    const cpx<T> V0 = a.H ? sample_point_sym(b, a.u[i], a.v[i], oob) : sample_point(b, a.u[i], a.v[i], oob);
+    T ur = a.u[i], vr = a.v[i];
+    if (a.rotate) {
+        ur = a.u[i] * a.cosPA - a.v[i] * a.sinPA;
+        vr = a.u[i] * a.sinPA + a.v[i] * a.cosPA;
+    }
+    cpx<T> V{T(0), T(0)};
+    for (int k = 0; k < nsrc; ++k) {
+        const T f = src[3 * k], dRA = src[3 * k + 1], dDec = src[3 * k + 2];
+        T dRAr = dRA, dDecr = dDec;
         if (a.rotate) {
-            ur = a.u[i] * a.cosPA - a.v[i] * a.sinPA;
-            vr = a.u[i] * a.sinPA + a.v[i] * a.cosPA;
+            dRAr = dRA * a.cosPA - dDec * a.sinPA;
+            dDecr = dRA * a.sinPA + dDec * a.cosPA;
         }
-        cpx<T> V{T(0), T(0)};
-        for (int k = 0; k < nsrc; ++k) {
-            const T f = src[3 * k], dRA = src[3 * k + 1], dDec = src[3 * k + 2];
-            T dRAr = dRA, dDecr = dDec;
-            if (a.rotate) {
-                dRAr = dRA * a.cosPA - dDec * a.sinPA;
-                dDecr = dRA * a.sinPA + dDec * a.cosPA;
-            }
-            cpx<T> Vk = V0;
-            if (!(dRAr == T(0) && dDecr == T(0))) {
-                const T ang = ur * T(double(dRAr) * (2. * double(T(M_PI)))) +
-                              vr * T(double(dDecr) * (2. * double(T(M_PI))));
-                T sn, cs;
-                fsincos(ang, &sn, &cs);
-                Vk = cpx<T>{V0.re * cs - V0.im * sn, V0.re * sn + V0.im * cs};
-            }
-            V.re += f * Vk.re;
-            V.im += f * Vk.im;
+        cpx<T> Vk = V0;
+        if (!(dRAr == T(0) && dDecr == T(0))) {
+            const T ang = ur * T(double(dRAr) * (2. * double(T(M_PI)))) +
+                          vr * T(double(dDecr) * (2. * double(T(M_PI))));
+            T sn, cs;
+            fsincos(ang, &sn, &cs);
+            Vk = cpx<T>{V0.re * cs - V0.im * sn, V0.re * sn + V0.im * cs};
         }
-        if (oob) V = cpx<T>{T(NAN), T(NAN)};
-        if constexpr (MODE == MODE_VIS) {
-            a.vis[a.perm ? a.perm[i] : i] = V;
-        } else {
-            const T sw = sqrt(a.w[i]);
-            const T cr = (V.re - a.re[i]) * sw;
-            const T ci = (V.im - a.im[i]) * sw;
-            if constexpr (std::is_same<T, double>::value) acc = cr * cr + ci * ci;
-            else acc = double(cr) * cr + double(ci) * ci;
+        V.re += f * Vk.re;
+        V.im += f * Vk.im;
+    }
+    if (oob) V = cpx<T>{T(NAN), T(NAN)};
+    return V;
+}
+
+// PPT blocks of 256 points per workgroup (more loads in flight per wave);
+// block q's chi2 partial is the same fixed-order sum as in k_sample.
+template <typename T, int MODE, int PPT>
+__global__ __launch_bounds__(256) void k_sample_multisrc(SampleArgs<T> a, int nsrc, const T* __restrict__ src,
+                                                         int nblocks) {
+    __shared__ double red[PPT][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double acc[PPT];
+    bool oob_any = false;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        acc[q] = 0.0;
+        const int i = (blockIdx.x * PPT + q) * 256 + tid;
+        if (i < a.nd) {
+            bool oob = false;
+            const cpx<T> V = multisrc_point(a, i, nsrc, src, oob);
+            oob_any |= oob;
+            if constexpr (MODE == MODE_VIS) {
+                a.vis[a.perm ? a.perm[i] : i] = V;
+            } else {
+                const T sw = sqrt(a.w[i]);
+                const T cr = (V.re - a.re[i]) * sw;
+                const T ci = (V.im - a.im[i]) * sw;
+                if constexpr (std::is_same<T, double>::value) acc[q] = cr * cr + ci * ci;
+                else acc[q] = double(cr) * cr + double(ci) * ci;
+            }
         }
     }
-    if (oob && a.flag) atomicOr(a.flag, 1u);
+    if (oob_any && a.flag) atomicOr(a.flag, 1u);
     if constexpr (MODE == MODE_CHI2) {
-        const double b = block_sum256(acc, red);
-        if (threadIdx.x == 0) a.partials[blockIdx.x] = b;
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const double x = wave_sum(acc[q]);
+            if (lane == 0) red[q][wv] = x;
+        }
+        __syncthreads();
+        if (tid < PPT) {
+            const int b = blockIdx.x * PPT + tid;
+            if (b < nblocks) a.partials[b] = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
+        }
     }
 }
 
