@@ -526,7 +526,7 @@ def main():
         port.threads(ncores)
         times = []
         tstart = time.perf_counter()
-        while time.perf_counter() - tstart < args.cpu_seconds and len(times) < 60:
+        while time.perf_counter() - tstart < args.cpu_seconds and len(times) < 400:
             t = time.perf_counter()
             c = port.chi2_profile(wl.intensity, wl.Rmin, wl.dR, wl.dxy, wl.nxy, wl.inc, wl.dRA, wl.dDec, wl.duv,
                                   wl.PA, wl.u, wl.v, wl.re, wl.im, wl.w, workers=ncores)
