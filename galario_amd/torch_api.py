@@ -172,25 +172,36 @@ def chi2_profile_multi(intensity, Rmin, dR, nxy, dxy, u, v, vis_obs_re, vis_obs_
 
 
 def _spread16(x: np.ndarray) -> np.ndarray:
-    x = x.astype(np.uint64) & 0xFFFF
-    x = (x | (x << 8)) & 0x00FF00FF
-    x = (x | (x << 4)) & 0x0F0F0F0F
-    x = (x | (x << 2)) & 0x33333333
-    x = (x | (x << 1)) & 0x55555555
+    """The 16 low bits of x moved to the even bit positions of a uint32."""
+    x = x.astype(np.uint32) & np.uint32(0xFFFF)
+    x = (x | (x << np.uint32(8))) & np.uint32(0x00FF00FF)
+    x = (x | (x << np.uint32(4))) & np.uint32(0x0F0F0F0F)
+    x = (x | (x << np.uint32(2))) & np.uint32(0x33333333)
+    x = (x | (x << np.uint32(1))) & np.uint32(0x55555555)
     return x
 
 
-def locality_order(u, v) -> np.ndarray:
-    """Permutation sorting (u, v) by the 32-bit Morton code of (|u|, sign(u) v)."""
+def locality_key(u, v) -> np.ndarray:
+    """The 32-bit Morton code of the quantised folded position (|u|, sign(u) v)."""
     u = np.asarray(u, dtype=np.float64)
     v = np.asarray(v, dtype=np.float64)
     s = np.copysign(1.0, u)
     fu, fv = np.abs(u), s * v
     r = max(float(np.max(np.abs(fu), initial=0.0)), float(np.max(np.abs(fv), initial=0.0)), 1e-300)
-    qu = np.clip(fu / r * 65535.0, 0, 65535).astype(np.uint64)
-    qv = np.clip((fv / r + 1.0) * 0.5 * 65535.0, 0, 65535).astype(np.uint64)
-    key = _spread16(qu) | (_spread16(qv) << np.uint64(1))
-    return np.argsort(key, kind='stable')
+    qu = np.clip(fu / r * 65535.0, 0, 65535).astype(np.uint32)
+    qv = np.clip((fv / r + 1.0) * 0.5 * 65535.0, 0, 65535).astype(np.uint32)
+    return _spread16(qu) | (_spread16(qv) << np.uint32(1))
+
+
+def locality_order(u, v) -> np.ndarray:
+    """Permutation sorting (u, v) by locality_key, stably: two stable passes over the
+    key's 16-bit halves, low then high (numpy sorts 16-bit integers by radix), which
+    give the permutation of a stable sort on the whole key."""
+    key = locality_key(u, v)
+    lo = (key & np.uint32(0xFFFF)).astype(np.uint16)
+    hi = (key >> np.uint32(16)).astype(np.uint16)
+    o = np.argsort(lo, kind='stable')
+    return o[np.argsort(hi[o], kind='stable')]
 
 
 class Dataset:

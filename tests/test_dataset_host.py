@@ -25,6 +25,22 @@ def test_locality_order_is_a_permutation_grouping_neighbours():
     assert np.median(step) < 0.05 * np.median(rand)
 
 
+def test_locality_order_is_the_stable_sort_of_the_key():
+    """The two 16-bit radix passes give the stable sort of the 32-bit Morton key,
+    ties (repeated points) included."""
+    from galario_amd.torch_api import locality_key, locality_order
+    u, v = _uv(20000, seed=1)
+    u[::7], v[::7] = u[3], v[3]
+    u[1::11] = -u[1::11]
+    key = locality_key(u, v)
+    assert key.dtype == np.uint32 and int(key.max()) >= 1 << 16
+    assert np.array_equal(locality_order(u, v), np.argsort(key.astype(np.uint64), kind='stable'))
+    # the key interleaves the bits, |u| on the even positions and the folded v on the odd ones:
+    # (u, v) = (1, 0) quantises to (65535, 32767), whose spread bits are 0x55555555 and 0x15555555
+    k0 = int(locality_key(np.array([1.0]), np.array([0.0]))[0])
+    assert k0 == 0x55555555 | (0x15555555 << 1)
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_dataset_extent_and_order_on_the_host(dtype):
     from galario_amd.torch_api import Dataset
