@@ -220,7 +220,9 @@ class Dataset:
         order = locality_order(u, v) if sort else np.arange(self.n)
         dev = torch.device(device)
         self.device, self.dtype = dev, dtype
-        put = lambda a: torch.from_numpy(np.ascontiguousarray(a[order])).to(dev, dtype)  # noqa: E731
+        order_t = torch.from_numpy(order.astype(np.int64)).to(dev)
+        # uploaded in the caller's order, gathered into the locality order on the device
+        put = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)[order_t].to(dtype).contiguous()  # noqa: E731
         self.u, self.v = put(u), put(v)
         # bound on sqrt(u^2 + v^2) of the stored values: the spectrum is built only within it
         us, vs = (a.astype(np.float32 if dtype == torch.float32 else np.float64).astype(np.float64) for a in (u, v))
@@ -231,7 +233,7 @@ class Dataset:
             if not (re.shape == im.shape == w.shape == u.shape):
                 raise ValueError('vis_obs_re, vis_obs_im, vis_obs_w must have the length of u')
             self.re, self.im, self.w = put(re), put(im), put(w)
-        self.order = torch.from_numpy(order.astype(np.int32)).to(dev)   # sorted position i <- caller's point order[i]
+        self.order = order_t.to(torch.int32)   # sorted position i <- caller's point order[i]
 
     def chi2_profile(self, intensity, Rmin, dR, nxy, dxy, dRA=0., dDec=0., PA=0., inc=0.,
                      check_central_pixel=True, out=None):
