@@ -233,7 +233,7 @@ def test_autograd_function():
                                           check_central_pixel=False)
     loss.backward()
     c, gI, gg = gpu_grad(d, ds=ds)
-    assert math.isclose(float(loss), 0.5 * c, rel_tol=1e-15)
+    assert math.isclose(float(loss.detach()), 0.5 * c, rel_tol=1e-15)
     assert np.linalg.norm(I.grad.cpu().numpy() - 0.5 * gI) <= 1e-12 * np.linalg.norm(gI)
     np.testing.assert_allclose(geom.grad.numpy(), 0.5 * gg, rtol=1e-12)
     with torch.no_grad():
