@@ -87,6 +87,14 @@ void check_input_xy(int nx, int ny) {
 void check_nd(int nd) {
     if (nd < 0) invalid("check input", "number of (u, v) points = " + std::to_string(nd) + " is negative");
 }
+// Device arrays of a _dev call: the profile always, u and v (and the observation when `obs`)
+// whenever there are points.  Runs after the size and geometry checks.
+template <typename T>
+void check_dev_inputs(const T* I, int nd, const T* u, const T* v, bool obs, const T* re, const T* im, const T* w) {
+    if (!I) invalid("check input", "null intensity pointer");
+    if (nd > 0 && (!u || !v)) invalid("check input", "null u or v pointer");
+    if (nd > 0 && obs && (!re || !im || !w)) invalid("check input", "null observed-visibility or weight pointer");
+}
 
 // ------------------------------------------------------------------ memory
 struct DevBuf {
@@ -934,6 +942,8 @@ int chi2_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int 
         check_nd(nd);
         ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
         SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        check_dev_inputs(I, nd, u, v, true, re, im, wt);
+        if (!chi2_d) invalid("check input", "null chi2 pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
@@ -952,6 +962,8 @@ int sample_profile_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, in
         check_nd(nd);
         ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
         SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        check_dev_inputs<T>(I, nd, u, v, false, nullptr, nullptr, nullptr);
+        if (nd > 0 && !vis_d) invalid("check input", "null visibility pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
@@ -976,6 +988,8 @@ int profile_multisrc_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, 
         if (nsrc > 0 && !src) invalid("multiple sources", "null source array");
         ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
         SampleSetup<T> ss = setup_sample<T>(PA, T(0), T(0));
+        check_dev_inputs(I, nd, u, v, chi2_d != nullptr, re, im, wt);
+        if (!chi2_d && nd > 0 && !vis_d) invalid("check input", "null visibility pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
@@ -1022,10 +1036,11 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
             invalid("chi2 gradient", "nxy = " + std::to_string(nxy) +
                                          " is not supported by the gradient (power of two from 128 to " +
                                          std::to_string(1 << max_fast_log2<T>()) + ")");
-        if (!grad_I || !grad_geom || !chi2_d) invalid("chi2 gradient", "null output pointer");
         if (nr > ADJROW_LDS_NR)
             invalid("chi2 gradient", "nr = " + std::to_string(nr) + " profile bins (the gradient supports up to " +
                                          std::to_string(ADJROW_LDS_NR) + ")");
+        if (!grad_I || !grad_geom || !chi2_d) invalid("chi2 gradient", "null output pointer");
+        check_dev_inputs(I, nd, u, v, true, re, im, wt);
         hipStream_t s = static_cast<hipStream_t>(stream);
         Workspace& w = workspace(s, false);
         ps.sw.I = I;
@@ -1551,9 +1566,9 @@ int galario_cleanup(void) {
             }
         }
         t_ws.clear();
-        std::lock_guard<std::mutex> lock(g_tw_mu);
-        for (auto& kv : g_tw) (void)hipFree(kv.second);
-        g_tw.clear();
+        // The twiddle tables stay: they are shared by every thread and baked into the
+        // kernel arguments of captured batch graphs, so they live until the process ends
+        // (a few hundred KB per device).
     });
 }
 
@@ -1769,9 +1784,9 @@ int galario_batch_take_oob_flag(void* plan, int* out) {
 
 int galario_read_probe(void* stream, const void* buf, long long bytes, int grid, double* out) {
     return guard([&] {
-        if (!buf || !out) invalid("read_probe", "null buffer");
         if (bytes < 0 || bytes % 16 != 0) invalid("read_probe", "bytes must be a non-negative multiple of 16");
         if (grid < 1 || grid > 65536) invalid("read_probe", "grid must be in [1, 65536]");
+        if (!buf || !out) invalid("read_probe", "null buffer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         hipLaunchKernelGGL(k_read_probe, dim3(grid), dim3(256), 0, s, static_cast<const cpx<double>*>(buf),
                            size_t(bytes / 16), out);

@@ -46,7 +46,9 @@ const char *galario_last_error(void);
 /* replaces galario::init() (src/galario.h:52; python/__init_module__.py.in:22) */
 int galario_init(void);
 /* replaces galario::cleanup() (src/galario.h:53): frees the calling thread's
- * workspaces and the twiddle tables */
+ * workspaces.  The FFT twiddle tables (shared by all threads and by captured
+ * batch graphs) are kept until the process ends, so batch plans and other
+ * threads' calls stay valid across a cleanup. */
 int galario_cleanup(void);
 /* replaces galario::threads(int) (src/galario.h:54): returns the setting
  * (default 16, kept for API compatibility; the kernels' launch shapes are fixed

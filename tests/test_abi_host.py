@@ -136,7 +136,10 @@ def test_batch_plan_argument_checks_before_any_gpu_work():
     create = _lib.lib.galario_batch_create_f64
     plan = ctypes.c_void_p()
 
-    def ds(nxy, dxy, nd=10, ptr=0x1000):
+    # No device pointer is ever passed: the datasets have no points (null arrays are valid then), except the
+    # one whose null arrays are the error under test.  A check that regressed would therefore fail on a later
+    # host check or plan no work, never hand a kernel a bogus address.
+    def ds(nxy, dxy, nd=0, ptr=None):
         return _lib.DatasetF64(nxy, dxy, 1.0 / (nxy * dxy), nd, ptr, ptr, ptr, ptr, ptr)
 
     def status(sets, nr=100, Rmin=1e-4 * a, dR=0.0005 * a, max_w=4, cc=0):
@@ -150,7 +153,7 @@ def test_batch_plan_argument_checks_before_any_gpu_work():
     with pytest.raises(ValueError, match="is odd"):
         _lib.check(status([ds(1023, 0.01 * a)]))
     with pytest.raises(ValueError, match="null array"):
-        _lib.check(status([ds(512, 0.01 * a, ptr=0)]))
+        _lib.check(status([ds(512, 0.01 * a, nd=10)]))
     with pytest.raises(ValueError, match=r"Expect \(dxy/2-Rmin\)/dR > 5"):
         _lib.check(status([ds(512, 0.01 * a)], dR=0.002 * a, cc=1))
     with pytest.raises(ValueError, match="max_walkers"):
@@ -160,29 +163,61 @@ def test_batch_plan_argument_checks_before_any_gpu_work():
 
 
 def test_gradient_argument_checks_before_any_gpu_work():
-    """The gradient entry point rejects unsupported sizes on the host."""
+    """The gradient entry point rejects unsupported sizes on the host.  Every array is null: the size checks
+    come first, and null arrays are themselves rejected before any GPU work."""
     a = galario_amd.arcsec
     f = _lib.lib.galario_chi2_profile_grad_dev_f64
-    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p] + [ctypes.c_double] * 3 + [ctypes.c_int] + \
-        [ctypes.c_double] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 3 + \
-        [ctypes.c_double]
-    p = ctypes.c_void_p(0x1000)
+    p = None
     common = (0.01 * a, 1024, 0.5, 0.0, 0.0, 1 / (1024 * 0.01 * a), 0.3, 10, p, p, p, p, p, 0, p, p, p, 0.0)
     with pytest.raises(ValueError, match="up to 2048"):
         _lib.check(f(None, 3000, p, 1e-4 * a, 0.0005 * a, *common))
     with pytest.raises(ValueError, match="not supported by the gradient"):
         _lib.check(f(None, 1000, p, 1e-4 * a, 0.0005 * a, 0.01 * a, 1000, *common[2:]))
+    with pytest.raises(ValueError, match="null output pointer"):
+        _lib.check(f(None, 1000, p, 1e-4 * a, 0.0005 * a, *common))
+
+
+def test_device_entry_points_reject_null_arrays():
+    """The _dev forms reject null device arrays on the host (after their size checks)."""
+    a = galario_amd.arcsec
+    f = _lib.lib.galario_chi2_profile_dev_f64
+    args = (None, 1000, None, 1e-4 * a, 0.0005 * a, 0.01 * a, 1024, 0.5, 0.0, 0.0, 1 / (1024 * 0.01 * a), 0.3, 10,
+            None, None, None, None, None, 0, None, 0.0)
+    with pytest.raises(ValueError, match="null intensity pointer"):
+        _lib.check(f(*args))
+    with pytest.raises(ValueError, match="is odd"):
+        _lib.check(f(*args[:6], 1023, *args[7:]))
+    g = _lib.lib.galario_sample_profile_dev_f64
+    with pytest.raises(ValueError, match="null intensity pointer"):
+        _lib.check(g(None, 1000, None, 1e-4 * a, 0.0005 * a, 0.01 * a, 1024, 0.5, 0.0, 0.0, 1 / (1024 * 0.01 * a),
+                     0.3, 10, None, None, 0, None, None, 0.0))
 
 
 def test_read_probe_argument_checks_before_any_gpu_work():
     """The read-rate reference rejects bad sizes and null buffers on the host."""
     f = _lib.lib.galario_read_probe
-    p = ctypes.c_void_p(0x1000)
     with pytest.raises(ValueError, match="null buffer"):
-        _lib.check(f(None, None, 16, 1, p))
+        _lib.check(f(None, None, 16, 1, None))
     with pytest.raises(ValueError, match="multiple of 16"):
-        _lib.check(f(None, p, 24, 1, p))
+        _lib.check(f(None, None, 24, 1, None))
     with pytest.raises(ValueError, match="grid must be"):
-        _lib.check(f(None, p, 32, 0, p))
+        _lib.check(f(None, None, 32, 0, None))
     with pytest.raises(ValueError, match="grid must be"):
-        _lib.check(f(None, p, 32, 70000, p))
+        _lib.check(f(None, None, 32, 70000, None))
+
+
+def test_galario_import_path():
+    """The package name and layout of galario (python/__init__.py.in:20-31, docs/quickstart.rst:72,85,144-145):
+    ``galario.double`` / ``galario.single`` are the HIP-backed modules themselves."""
+    import galario
+    import galario.double
+    import galario.single
+    from galario import arcsec, au, cgs_to_Jy, deg, pc
+    from galario.double import chi2Profile, get_image_size, sampleProfile  # noqa: F401
+    assert galario.double is galario_amd.double and galario.single is galario_amd.single
+    assert galario.__version__ == '1.2.2' and galario.HAVE_CUDA is False
+    assert (arcsec, deg) == (4.84813681109536e-06, 0.017453292519943295)
+    assert (au, pc, cgs_to_Jy) == (galario_amd.double.au, galario_amd.double.pc, 1e23)
+    assert chi2Profile is galario_amd.double.chi2Profile
+    for mod in (galario.double, galario.single):
+        assert set(mod.__all__) >= {'sampleImage', 'chi2Image', 'sampleProfile', 'chi2Profile', 'get_image_size'}

@@ -173,3 +173,24 @@ def test_batch_out_of_grid_point():
     with pytest.raises(ValueError, match="nwalkers"):
         plan(torch.cat([I, I]), torch.cat([geom, geom]))
     plan.close()
+
+
+def test_batch_plan_survives_cleanup():
+    """galario's _cleanup() (a public API) between two calls of one plan: the plan's captured graph keeps
+    working with the same results (the FFT twiddle tables it reads live until the process ends)."""
+    import galario_amd
+    from galario_amd import torch_api as ta
+    sets, _ = make_sets(specs=((256, 3000, 0.02), (512, 8000, 0.01)))
+    I, geom, Rmin, dR = walkers(3)
+    plan = ta.BatchedChi2Profile(sets, I.shape[1], Rmin, dR, max_walkers=3, check_central_pixel=False)
+    c1 = plan(I, geom).cpu().numpy()
+    _, _, ready = plan.info(3)
+    assert ready
+    torch.cuda.synchronize()
+    galario_amd.double._cleanup()
+    c2 = plan(I, geom).cpu().numpy()
+    assert np.array_equal(c1, c2)
+    # a plain call after the cleanup rebuilds its workspace and agrees with the plan
+    ref = np.array(plain(sets, I, geom, Rmin, dR)).sum(axis=1)
+    np.testing.assert_allclose(c2, ref, rtol=1e-12)
+    plan.close()
