@@ -48,12 +48,13 @@ class DatasetF64(ctypes.Structure):
 
 class DatasetF32(ctypes.Structure):
     """galario_dataset_f32 (include/galario_amd.h)."""
-    _fields_ = [('nxy', ctypes.c_int), ('dxy', ctypes.c_float), ('duv', ctypes.c_float), ('nd', ctypes.c_int),
+    _fields_ = [('nxy', ctypes.c_int), ('dxy', ctypes.c_float), ('duv', ctypes.c_double), ('nd', ctypes.c_int),
                 ('u', _vp), ('v', _vp), ('vis_obs_re', _vp), ('vis_obs_im', _vp), ('weights', _vp),
                 ('uvmax', ctypes.c_double)]
 
 
 BATCH_EAGER = 1
+BATCH_MODE_GRAD, BATCH_MODE_F64_PHASE = 1, 2
 
 
 def _load():
@@ -81,6 +82,8 @@ def _load():
     lib.galario_batch_info.argtypes = [_vp, _c_int, _c_int, _ip, _ip, _ip]
     lib.galario_batch_set_chunk.argtypes = [_vp, _c_int, _c_int, _c_int, _c_int]
     lib.galario_batch_take_oob_flag.argtypes = [_vp, _ip]
+    lib.galario_batch_chi2_profile_f32x.argtypes = [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _c_int]
+    lib.galario_batch_chi2_profile_grad_f32x.argtypes = [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int]
     for suf, real, ds in (('f64', ctypes.c_double, DatasetF64), ('f32', ctypes.c_float, DatasetF32)):
         R = real
         f = getattr(lib, f'galario_batch_create_{suf}')

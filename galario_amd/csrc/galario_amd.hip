@@ -1106,6 +1106,9 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
 // buffers, launch the graph and copy the results out, all on the caller's
 // stream.
 constexpr uint32_t BATCH_MAGIC = 0x67616c62u;
+// Launch-sequence modes of a plan: bit 0 gradient, bit 1 FP32 bulk with FP64 geometry, phase and
+// per-point arithmetic (float plans only).
+constexpr int MODE_GRAD = 1, MODE_F64_PHASE = 2;
 constexpr size_t BATCH_SLOT_BUDGET = size_t(16) << 30;   // bytes of slot buffers at most
 
 struct BatchPlanBase {
@@ -1121,6 +1124,7 @@ struct BatchSet {
     int nd;
     const T *u, *v, *re, *im, *w;
     double uvmax;          // <= 0: not given
+    double duv_d;          // duv in double (the FP32-bulk form's FP64 per-point arithmetic)
 };
 
 template <typename T>
@@ -1155,7 +1159,7 @@ struct BatchPlan : BatchPlanBase {
         int kw = 4;                    // walkers per sample workgroup (forward)
         hipGraphExec_t exec = nullptr;
     };
-    std::map<std::pair<int, int>, Exec> runs;   // (nwalkers, grad)
+    std::map<std::pair<int, int>, Exec> runs;   // (nwalkers, mode)
     std::mutex mu;
 
     ~BatchPlan() override {
@@ -1240,25 +1244,35 @@ BatchPtrs<T> batch_ptrs(BatchPlan<T>& P, bool grad) {
 
 // Enqueue one batched evaluation of W walkers (parameters already in P.I /
 // P.geom) in chunks of `chunk` walkers; only kernel launches (capturable).
-template <typename T>
+template <typename T, bool MIX>
 void launch_sample_multi(int kw, dim3 grid, hipStream_t s, const SampleArgs<T>& a, const BatchPtrs<T>& b, int nw) {
     switch (kw) {
-        case 1: hipLaunchKernelGGL((k_sample_multi_b<T, 1>), grid, dim3(256), 0, s, a, b, nw); break;
-        case 2: hipLaunchKernelGGL((k_sample_multi_b<T, 2>), grid, dim3(256), 0, s, a, b, nw); break;
-        case 4: hipLaunchKernelGGL((k_sample_multi_b<T, 4>), grid, dim3(256), 0, s, a, b, nw); break;
-        case 8: hipLaunchKernelGGL((k_sample_multi_b<T, 8>), grid, dim3(256), 0, s, a, b, nw); break;
+        case 1: hipLaunchKernelGGL((k_sample_multi_b<T, 1, MIX>), grid, dim3(256), 0, s, a, b, nw); break;
+        case 2: hipLaunchKernelGGL((k_sample_multi_b<T, 2, MIX>), grid, dim3(256), 0, s, a, b, nw); break;
+        case 4: hipLaunchKernelGGL((k_sample_multi_b<T, 4, MIX>), grid, dim3(256), 0, s, a, b, nw); break;
+        case 8: hipLaunchKernelGGL((k_sample_multi_b<T, 8, MIX>), grid, dim3(256), 0, s, a, b, nw); break;
         default: fail(GALARIO_ERR_RUNTIME, "internal: walkers per sample workgroup must be 1, 2, 4 or 8");
     }
 }
 
 template <typename T>
-void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, int kw) {
+void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, int mode, int kw) {
     const int nr = P.nr;
+    constexpr bool is_f32 = std::is_same<T, float>::value;   // (the mixed form exists for float plans only)
+    const bool grad = mode & MODE_GRAD, mix = is_f32 && (mode & MODE_F64_PHASE);
     const size_t nz = std::max(size_t(W), grad ? size_t(W) * size_t(std::max(nr, 4)) : size_t(W));
-    hipLaunchKernelGGL((k_batch_setup<T>), dim3(blocks_for(nz, 256)), dim3(256), 0, s, W,
-                       static_cast<const T*>(P.geom.p), static_cast<WalkerPrm<T>*>(P.wp.p),
-                       grad ? static_cast<double*>(P.gI.p) : nullptr, nr,
-                       grad ? static_cast<double*>(P.ggeom.p) : nullptr);
+    if constexpr (is_f32) {
+        if (mix)
+            hipLaunchKernelGGL((k_batch_setup<T, double>), dim3(blocks_for(nz, 256)), dim3(256), 0, s, W,
+                               static_cast<const double*>(P.geom.p), static_cast<WalkerPrm<T>*>(P.wp.p),
+                               grad ? static_cast<double*>(P.gI.p) : nullptr, nr,
+                               grad ? static_cast<double*>(P.ggeom.p) : nullptr);
+    }
+    if (!mix)
+        hipLaunchKernelGGL((k_batch_setup<T, T>), dim3(blocks_for(nz, 256)), dim3(256), 0, s, W,
+                           static_cast<const T*>(P.geom.p), static_cast<WalkerPrm<T>*>(P.wp.p),
+                           grad ? static_cast<double*>(P.gI.p) : nullptr, nr,
+                           grad ? static_cast<double*>(P.ggeom.p) : nullptr);
     BatchPtrs<T> b = batch_ptrs(P, grad);
     unsigned* flag = static_cast<unsigned*>(P.flag.p);
     for (int w0 = 0; w0 < W; w0 += chunk) {
@@ -1294,11 +1308,15 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, 
                 a.w = d.w;
                 a.flag = flag;
                 a.rho2 = G.ex.rho2;
+                a.duv_d = d.duv_d;
                 b.partOff = P.off[si];
-                if (grad)
-                    hipLaunchKernelGGL((k_sample_grad_sym_b<T>), dim3(P.nb[si], nw), dim3(256), 0, s, a, b);
-                else
-                    launch_sample_multi<T>(kw, dim3(P.nb[si], (nw + kw - 1) / kw), s, a, b, nw);
+                const dim3 gs(P.nb[si], nw), gm(P.nb[si], (nw + kw - 1) / kw);
+                if constexpr (is_f32) {
+                    if (grad && mix) hipLaunchKernelGGL((k_sample_grad_sym_b<T, true>), gs, dim3(256), 0, s, a, b);
+                    else if (mix) launch_sample_multi<T, true>(kw, gm, s, a, b, nw);
+                }
+                if (grad && !mix) hipLaunchKernelGGL((k_sample_grad_sym_b<T, false>), gs, dim3(256), 0, s, a, b);
+                else if (!mix) launch_sample_multi<T, false>(kw, gm, s, a, b, nw);
             }
             if (grad) {
                 AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0, nullptr, 0, Dst4{}, ablocks,
@@ -1399,7 +1417,7 @@ int batch_create(int nsets, const BatchSet<T>* sets, int nr, T Rmin, T dR, int m
         P->sTh = sTh;
         // parameters, walker scalars, partials, outputs
         P->I.template get<T>(size_t(max_w) * nr);
-        P->geom.template get<T>(size_t(max_w) * 4);
+        P->geom.template get<double>(size_t(max_w) * 4);   // T, or double geometry (MODE_F64_PHASE)
         P->wp.template get<WalkerPrm<T>>(size_t(max_w));
         P->part.template get<double>(size_t(max_w) * 4 * P->ldPart);
         P->out_sets.template get<double>(size_t(max_w) * nsets);
@@ -1429,15 +1447,15 @@ BatchPlan<T>& plan_of(void* h) {
 
 // Average time of one eager evaluation at `chunk` (after one warm-up).
 template <typename T>
-float batch_time(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, int kw) {
-    P.ensure_slots(chunk, grad, s);
-    batch_enqueue(P, s, W, chunk, grad, kw);
+float batch_time(BatchPlan<T>& P, hipStream_t s, int W, int chunk, int mode, int kw) {
+    P.ensure_slots(chunk, mode & MODE_GRAD, s);
+    batch_enqueue(P, s, W, chunk, mode, kw);
     hipEvent_t e0, e1;
     HC(hipEventCreate(&e0));
     HC(hipEventCreate(&e1));
     HC(hipEventRecord(e0, s));
     const int reps = 2;
-    for (int r = 0; r < reps; ++r) batch_enqueue(P, s, W, chunk, grad, kw);
+    for (int r = 0; r < reps; ++r) batch_enqueue(P, s, W, chunk, mode, kw);
     HC(hipEventRecord(e1, s));
     HC(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1452,7 +1470,8 @@ float batch_time(BatchPlan<T>& P, hipStream_t s, int W, int chunk, bool grad, in
 // budget) at 4 walkers per sample workgroup, then (forward) the walkers per
 // sample workgroup (1, 2, 4, 8) at that chunk size.
 template <typename T>
-void batch_tune(BatchPlan<T>& P, hipStream_t s, int W, bool grad, int* chunk, int* kw) {
+void batch_tune(BatchPlan<T>& P, hipStream_t s, int W, int mode, int* chunk, int* kw) {
+    const bool grad = mode & MODE_GRAD;
     const int cmax = P.max_chunk(grad, W);
     std::vector<int> cand;
     for (int c = 1; c < cmax; c *= 2) cand.push_back(c);
@@ -1460,7 +1479,7 @@ void batch_tune(BatchPlan<T>& P, hipStream_t s, int W, bool grad, int* chunk, in
     int best = cand.back(), bkw = 4;
     float tbest = 0.f;
     for (int c : cand) {
-        const float t = batch_time(P, s, W, c, grad, 4);
+        const float t = batch_time(P, s, W, c, mode, 4);
         if (c == cand.front() || t < tbest) {
             tbest = t;
             best = c;
@@ -1468,7 +1487,7 @@ void batch_tune(BatchPlan<T>& P, hipStream_t s, int W, bool grad, int* chunk, in
     }
     if (!grad)
         for (int k : {1, 2, 8}) {
-            const float t = batch_time(P, s, W, best, grad, k);
+            const float t = batch_time(P, s, W, best, mode, k);
             if (t < tbest) {
                 tbest = t;
                 bkw = k;
@@ -1478,12 +1497,17 @@ void batch_tune(BatchPlan<T>& P, hipStream_t s, int W, bool grad, int* chunk, in
     *kw = bkw;
 }
 
-template <typename T>
-int batch_run(void* h, void* stream, int W, const T* I, const T* geom, double* chi2_sets, double* chi2,
+// geom of type G: T, or double with MODE_F64_PHASE (float plans only).
+template <typename T, typename G>
+int batch_run(void* h, void* stream, int W, const T* I, const G* geom, double* chi2_sets, double* chi2,
               double* gI, double* ggeom, int flags, bool grad) {
     return guard([&] {
         BatchPlan<T>& P = plan_of<T>(h);
         std::lock_guard<std::mutex> lk(P.mu);
+        constexpr bool mix = !std::is_same<G, T>::value;
+        static_assert(!mix || (std::is_same<T, float>::value && std::is_same<G, double>::value),
+                      "FP64 geometry is the float plans' mixed form");
+        const int mode = (grad ? MODE_GRAD : 0) | (mix ? MODE_F64_PHASE : 0);
         if (W < 1 || W > P.max_w)
             invalid("batch", "nwalkers = " + std::to_string(W) + " (plan holds 1 to " + std::to_string(P.max_w) + ")");
         if (!I || !geom) invalid("batch", "null parameter array");
@@ -1501,13 +1525,13 @@ int batch_run(void* h, void* stream, int W, const T* I, const T* geom, double* c
         if (I != P.I.p)
             HC(hipMemcpyAsync(P.I.p, I, sizeof(T) * size_t(W) * P.nr, hipMemcpyDeviceToDevice, s));
         if (geom != P.geom.p)
-            HC(hipMemcpyAsync(P.geom.p, geom, sizeof(T) * size_t(W) * 4, hipMemcpyDeviceToDevice, s));
-        const auto key = std::make_pair(W, grad ? 1 : 0);
+            HC(hipMemcpyAsync(P.geom.p, geom, sizeof(G) * size_t(W) * 4, hipMemcpyDeviceToDevice, s));
+        const auto key = std::make_pair(W, mode);
         auto& run = P.runs[key];
-        if (run.chunk == 0) batch_tune(P, s, W, grad, &run.chunk, &run.kw);
+        if (run.chunk == 0) batch_tune(P, s, W, mode, &run.chunk, &run.kw);
         P.ensure_slots(run.chunk, grad, s);
         if (flags & GALARIO_BATCH_EAGER) {
-            batch_enqueue(P, s, W, run.chunk, grad, run.kw);
+            batch_enqueue(P, s, W, run.chunk, mode, run.kw);
         } else {
             if (!run.exec) {
                 // capture on the plan's stream (nothing runs during capture)
@@ -1515,7 +1539,7 @@ int batch_run(void* h, void* stream, int W, const T* I, const T* geom, double* c
                 hipGraph_t graph = nullptr;
                 HC(hipStreamBeginCapture(P.cap, hipStreamCaptureModeThreadLocal));
                 try {
-                    batch_enqueue(P, P.cap, W, run.chunk, grad, run.kw);
+                    batch_enqueue(P, P.cap, W, run.chunk, mode, run.kw);
                 } catch (...) {
                     (void)hipStreamEndCapture(P.cap, &graph);
                     if (graph) (void)hipGraphDestroy(graph);
@@ -1692,23 +1716,34 @@ GL_BOTH(GL_MULTI)
         std::vector<BatchSet<T>> v;                                                                               \
         if (sets && nsets > 0 && nsets <= BATCH_MAX_SETS)                                                         \
             for (int i = 0; i < nsets; ++i)                                                                       \
-                v.push_back(BatchSet<T>{sets[i].nxy, sets[i].dxy, sets[i].duv, sets[i].nd, sets[i].u, sets[i].v, \
+                v.push_back(BatchSet<T>{sets[i].nxy, sets[i].dxy, T(sets[i].duv), sets[i].nd, sets[i].u, sets[i].v, \
                                         sets[i].vis_obs_re, sets[i].vis_obs_im, sets[i].weights,                 \
-                                        sets[i].uvmax});                                                          \
+                                        sets[i].uvmax, double(sets[i].duv)});                                     \
         return batch_create<T>(nsets, v.empty() ? nullptr : v.data(), nr, Rmin, dR, max_walkers, check_central,  \
                                plan);                                                                             \
     }                                                                                                             \
     int galario_batch_chi2_profile_##SUF(void* plan, void* stream, int nwalkers, const T* I, const T* geom,      \
                                          double* chi2_sets, double* chi2, int flags) {                            \
-        return batch_run<T>(plan, stream, nwalkers, I, geom, chi2_sets, chi2, nullptr, nullptr, flags, false);    \
+        return batch_run<T, T>(plan, stream, nwalkers, I, geom, chi2_sets, chi2, nullptr, nullptr, flags, false); \
     }                                                                                                             \
     int galario_batch_chi2_profile_grad_##SUF(void* plan, void* stream, int nwalkers, const T* I, const T* geom, \
                                               double* chi2_sets, double* chi2, double* grad_I, double* grad_geom, \
                                               int flags) {                                                        \
-        return batch_run<T>(plan, stream, nwalkers, I, geom, chi2_sets, chi2, grad_I, grad_geom, flags, true);    \
+        return batch_run<T, T>(plan, stream, nwalkers, I, geom, chi2_sets, chi2, grad_I, grad_geom, flags, true); \
     }
 
 GL_BOTH(GL_BATCH)
+
+int galario_batch_chi2_profile_f32x(void* plan, void* stream, int nwalkers, const float* I, const double* geom,
+                                    double* chi2_sets, double* chi2, int flags) {
+    return batch_run<float, double>(plan, stream, nwalkers, I, geom, chi2_sets, chi2, nullptr, nullptr, flags, false);
+}
+
+int galario_batch_chi2_profile_grad_f32x(void* plan, void* stream, int nwalkers, const float* I, const double* geom,
+                                         double* chi2_sets, double* chi2, double* grad_I, double* grad_geom,
+                                         int flags) {
+    return batch_run<float, double>(plan, stream, nwalkers, I, geom, chi2_sets, chi2, grad_I, grad_geom, flags, true);
+}
 
 int galario_batch_destroy(void* plan) {
     return guard([&] {
@@ -1719,17 +1754,17 @@ int galario_batch_destroy(void* plan) {
     });
 }
 
-int galario_batch_info(void* plan, int nwalkers, int grad, int* chunk, int* walkers_per_sample, int* graph_ready) {
+int galario_batch_info(void* plan, int nwalkers, int mode, int* chunk, int* walkers_per_sample, int* graph_ready) {
     return guard([&] {
         auto* b = static_cast<BatchPlanBase*>(plan);
         if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
         int c = 0, k = 0, g = 0;
         auto look = [&](auto& P) {
             std::lock_guard<std::mutex> lk(P.mu);
-            auto it = P.runs.find(std::make_pair(nwalkers, grad ? 1 : 0));
+            auto it = P.runs.find(std::make_pair(nwalkers, mode));
             if (it != P.runs.end()) {
                 c = it->second.chunk;
-                k = grad ? 1 : it->second.kw;
+                k = (mode & MODE_GRAD) ? 1 : it->second.kw;
                 g = it->second.exec != nullptr;
             }
         };
@@ -1741,20 +1776,23 @@ int galario_batch_info(void* plan, int nwalkers, int grad, int* chunk, int* walk
     });
 }
 
-int galario_batch_set_chunk(void* plan, int nwalkers, int grad, int chunk, int walkers_per_sample) {
+int galario_batch_set_chunk(void* plan, int nwalkers, int mode, int chunk, int walkers_per_sample) {
     return guard([&] {
         auto* b = static_cast<BatchPlanBase*>(plan);
         if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
         auto set = [&](auto& P) {
             std::lock_guard<std::mutex> lk(P.mu);
             if (nwalkers < 1 || nwalkers > P.max_w) invalid("batch", "nwalkers out of range");
-            if (chunk < 1 || chunk > P.max_chunk(grad != 0, nwalkers))
+            if (mode < 0 || mode > (MODE_GRAD | MODE_F64_PHASE) || ((mode & MODE_F64_PHASE) && P.prec != 4))
+                invalid("batch", "mode = " + std::to_string(mode) + " (gradient 1, FP64 phase 2 on float plans)");
+            const bool grad = mode & MODE_GRAD;
+            if (chunk < 1 || chunk > P.max_chunk(grad, nwalkers))
                 invalid("batch", "chunk = " + std::to_string(chunk) + " (expect 1 to " +
-                                     std::to_string(P.max_chunk(grad != 0, nwalkers)) + ")");
+                                     std::to_string(P.max_chunk(grad, nwalkers)) + ")");
             if (walkers_per_sample != 1 && walkers_per_sample != 2 && walkers_per_sample != 4 &&
                 walkers_per_sample != 8)
                 invalid("batch", "walkers per sample workgroup must be 1, 2, 4 or 8");
-            auto& run = P.runs[std::make_pair(nwalkers, grad ? 1 : 0)];
+            auto& run = P.runs[std::make_pair(nwalkers, mode)];
             if (run.exec) (void)hipGraphExecDestroy(run.exec);
             run.exec = nullptr;
             run.chunk = chunk;

@@ -78,8 +78,12 @@ __device__ __forceinline__ int wave_max(int x) {
     return x;
 }
 
-template <typename T>
-__device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, const GradArgs<T>& g, int blk) {
+// Storage T (H and the dataset streams of `a`), arithmetic P with the scalars of `sc` (P = T, sc = a:
+// galario's arithmetic in that precision; T = float, P = double: the FP32-bulk form with FP64
+// per-point arithmetic).  The reverse mode runs in double either way.
+template <typename T, typename P>
+__device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, const SampleArgs<P>& sc,
+                                                     const GradArgs<T>& g, int blk) {
     __shared__ double win[GWIN];
     __shared__ double hot[GHOT * GHOT];
     __shared__ double red[4][4];
@@ -92,50 +96,50 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
     bool live = false, oob = false;
     for (int j = tid; j < GHOT * GHOT; j += 256) hot[j] = 0.0;
     if (i < a.nd) {
-        const T u0 = a.u[i], v0 = a.v[i];
-        const GridPos<T> p = grid_pos(a, u0, v0);
+        const P u0 = P(a.u[i]), v0 = P(a.v[i]);
+        const GridPos<P> p = grid_pos(sc, u0, v0);
         oob = p.oob;
-        const T q = p.q, t = p.t;
+        const P q = p.q, t = p.t;
         const int half = a.nrow / 2;
         cu0 = p.fl_u;
         cu1 = p.fu1;
         cm0 = abs(p.fl_v - half);
         cm1 = abs(p.fv1 - half);
-        const T y0 = a.H[h_index(cu0, cm0, a.ldH)], y1 = a.H[h_index(cu0, cm1, a.ldH)];
-        const T y3 = a.H[h_index(cu1, cm0, a.ldH)], y2 = a.H[h_index(cu1, cm1, a.ldH)];
-        const T z = (t * (q * ((y0 + y2) - (y1 + y3))) + (t * (y1 - y0) + q * (y3 - y0))) + y0;
-        const T a0 = fabs(y0), a1 = fabs(y1), a2 = fabs(y2), a3 = fabs(y3);
-        T amp = a0;
+        const P y0 = P(a.H[h_index(cu0, cm0, a.ldH)]), y1 = P(a.H[h_index(cu0, cm1, a.ldH)]);
+        const P y3 = P(a.H[h_index(cu1, cm0, a.ldH)]), y2 = P(a.H[h_index(cu1, cm1, a.ldH)]);
+        const P z = (t * (q * ((y0 + y2) - (y1 + y3))) + (t * (y1 - y0) + q * (y3 - y0))) + y0;
+        const P a0 = fabs(y0), a1 = fabs(y1), a2 = fabs(y2), a3 = fabs(y3);
+        P amp = a0;
         amp += (a3 - a0) * q;
         amp += (a1 - a0) * t;
         amp += (((a0 - a1) + a2) - a3) * t * q;
-        const T sg = copysign(T(1), z);
-        T cth = T(1), sth = T(0);
-        if (a.phase) {
-            const T ang = p.urot * a.dRA2 + p.vrot * a.dDec2;
+        const P sg = copysign(P(1), z);
+        P cth = P(1), sth = P(0);
+        if (sc.phase) {
+            const P ang = p.urot * sc.dRA2 + p.vrot * sc.dDec2;
             fsincos(ang, &sth, &cth);
         }
-        const T V0 = copysign(amp, z);
-        const T Vre = V0 * cth, Vim = V0 * sth;
-        const T w = a.w[i];
-        const T sw = sqrt(w);
-        const T cr = (Vre - a.re[i]) * sw, ci = (Vim - a.im[i]) * sw;
-        if constexpr (std::is_same<T, double>::value) c2 = cr * cr + ci * ci;
+        const P V0 = copysign(amp, z);
+        const P Vre = V0 * cth, Vim = V0 * sth;
+        const P w = P(a.w[i]), ore = P(a.re[i]), oim = P(a.im[i]);
+        const P sw = sqrt(w);
+        const P cr = (Vre - ore) * sw, ci = (Vim - oim) * sw;
+        if constexpr (std::is_same<P, double>::value) c2 = cr * cr + ci * ci;
         else c2 = double(cr) * cr + double(ci) * ci;
         // reverse mode in double
-        const double Gre = 2.0 * double(w) * (double(Vre) - double(a.re[i]));
-        const double Gim = 2.0 * double(w) * (double(Vim) - double(a.im[i]));
+        const double Gre = 2.0 * double(w) * (double(Vre) - double(ore));
+        const double Gim = 2.0 * double(w) * (double(Vim) - double(oim));
         const double gA = double(sg) * (Gre * double(cth) + Gim * double(sth));
         const double gth = Gim * double(Vre) - Gre * double(Vim);
         const double dq = double(1 - t) * double(a3 - a0) + double(t) * double(a2 - a1);
         const double dt = double(1 - q) * double(a1 - a0) + double(q) * double(a2 - a3);
-        const double s_duv = double(p.sign_u) / double(a.duv);
-        gpa = gA * (dq * s_duv * (-double(p.vrot)) + dt * double(a.v_origin) * s_duv * double(p.urot));
+        const double s_duv = double(p.sign_u) / double(sc.duv);
+        gpa = gA * (dq * s_duv * (-double(p.vrot)) + dt * double(sc.v_origin) * s_duv * double(p.urot));
         const double two_pi = 6.283185307179586476925286766559;
         gra = gth * two_pi * double(u0);
         gde = gth * two_pi * double(v0);
         if (!oob) {
-            const auto sgn = [](T y) { return double((y > T(0)) - (y < T(0))); };
+            const auto sgn = [](P y) { return double((y > P(0)) - (y < P(0))); };
             gk[0] = gA * (double(1 - t) * double(1 - q)) * sgn(y0);
             gk[1] = gA * (double(t) * double(1 - q)) * sgn(y1);
             gk[2] = gA * (double(1 - t) * double(q)) * sgn(y3);
@@ -238,7 +242,7 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_sample_grad_sym(SampleArgs<T> a, GradArgs<T> g) {
-    sample_grad_sym_body<T>(a, g, blockIdx.x);
+    sample_grad_sym_body<T, T>(a, a, g, blockIdx.x);
 }
 
 // Destinations of the 4 partial streams (chi2, dPA, ddRA, ddDec).

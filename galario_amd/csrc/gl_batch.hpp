@@ -27,19 +27,24 @@ struct WalkerPrm {
     int phase;
     T dRA2, dDec2;       // 2 pi dRArot, 2 pi dDecrot
     double tan_inc;
+    // the same rotation and phase scalars in double (the FP32-bulk form's FP64 per-point arithmetic)
+    double cosPAd, sinPAd, dRA2d, dDec2d;
+    int phased;
 };
 
-// geom: [W][4] = inc, PA, dRA, dDec.  Also clears the gradient accumulators
-// gI[0..W*nr) and ggeom[0..4W) when given.
-template <typename T>
-__global__ __launch_bounds__(256) void k_batch_setup(int W, const T* __restrict__ geom, WalkerPrm<T>* wp, double* gI,
+// geom: [W][4] = inc, PA, dRA, dDec, of type G (T, or double for the FP32-bulk form with FP64
+// geometry).  The T scalars follow the single call's host setup in T (setup_profile,
+// setup_sample); the double ones are the FP64 single call's, from the G values.  Also clears the
+// gradient accumulators gI[0..W*nr) and ggeom[0..4W) when given.
+template <typename T, typename G>
+__global__ __launch_bounds__(256) void k_batch_setup(int W, const G* __restrict__ geom, WalkerPrm<T>* wp, double* gI,
                                                      int nr, double* ggeom) {
     const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
     if (i < size_t(W)) {
-        const T inc = geom[4 * i], PA = geom[4 * i + 1], dRA = geom[4 * i + 2], dDec = geom[4 * i + 3];
+        const T PA = T(geom[4 * i + 1]), dRA = T(geom[4 * i + 2]), dDec = T(geom[4 * i + 3]);
         WalkerPrm<T> p;
-        p.cos_inc = T(cos(double(inc)));
-        p.tan_inc = tan(double(inc));
+        p.cos_inc = T(cos(double(geom[4 * i])));
+        p.tan_inc = tan(double(geom[4 * i]));
         T dRArot, dDecrot;
         if (PA == T(0)) {
             p.rotate = 0;
@@ -57,6 +62,19 @@ __global__ __launch_bounds__(256) void k_batch_setup(int W, const T* __restrict_
         p.phase = !(dRArot == T(0) && dDecrot == T(0));
         p.dRA2 = T(double(dRArot) * (2. * double(T(M_PI))));
         p.dDec2 = T(double(dDecrot) * (2. * double(T(M_PI))));
+        const double PAd = double(geom[4 * i + 1]), dRAd = double(geom[4 * i + 2]), dDecd = double(geom[4 * i + 3]);
+        double dRArotd = dRAd, dDecrotd = dDecd;
+        p.cosPAd = 1.0;
+        p.sinPAd = 0.0;
+        if (PAd != 0.0) {
+            p.cosPAd = cos(PAd);
+            p.sinPAd = sin(PAd);
+            dRArotd = dRAd * p.cosPAd - dDecd * p.sinPAd;
+            dDecrotd = dRAd * p.sinPAd + dDecd * p.cosPAd;
+        }
+        p.phased = !(dRArotd == 0.0 && dDecrotd == 0.0);
+        p.dRA2d = dRArotd * (2. * M_PI);
+        p.dDec2d = dDecrotd * (2. * M_PI);
         wp[i] = p;
     }
     if (gI && i < size_t(W) * size_t(nr)) gI[i] = 0.0;
@@ -117,11 +135,36 @@ __device__ __forceinline__ void walker_sample_args(SampleArgs<T>& a, const Batch
     a.dDec2 = p.dDec2;
 }
 
+// Scalars of the FP64 per-point arithmetic for walker w on the dataset of `a` (only the fields
+// grid_pos and apply_phase_pt read).
+template <typename T>
+__device__ __forceinline__ SampleArgs<double> walker_scalars_f64(const SampleArgs<T>& a, const BatchPtrs<T>& b,
+                                                                 int w) {
+    const WalkerPrm<T> p = b.wp[w];
+    SampleArgs<double> sc{};
+    sc.nrow = a.nrow;
+    sc.ncol = a.ncol;
+    sc.v_origin = double(a.v_origin);
+    sc.duv = a.duv_d;
+    sc.rho2 = a.rho2;
+    sc.rotate = p.rotate;
+    sc.cosPA = p.cosPAd;
+    sc.sinPA = p.sinPAd;
+    sc.phase = p.phased;
+    sc.dRA2 = p.dRA2d;
+    sc.dDec2 = p.dDec2d;
+    return sc;
+}
+
 // KW walkers per workgroup on the same points: the dataset streams (u, v, Re,
 // Im, w) are read once for KW walkers.  Each walker's per-workgroup partial
 // is the same fixed-order sum as in the one-walker kernel.
-template <typename T, int KW>
+//
+// MIX (T = float only): FP32 storage with the position, interpolation, phase and residual in FP64
+// (walker_scalars_f64).
+template <typename T, int KW, bool MIX = false>
 __global__ __launch_bounds__(256) void k_sample_multi_b(SampleArgs<T> a, BatchPtrs<T> b, int nw) {
+    using P = std::conditional_t<MIX, double, T>;
     __shared__ double red[KW][4];
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int s0 = blockIdx.y * KW;
@@ -130,9 +173,9 @@ __global__ __launch_bounds__(256) void k_sample_multi_b(SampleArgs<T> a, BatchPt
 #pragma unroll
     for (int k = 0; k < KW; ++k) acc[k] = 0.0;
     if (i < a.nd) {
-        const T u0 = a.u[i], v0 = a.v[i];
-        const T re = a.re[i], im = a.im[i];
-        const T sw = sqrt(a.w[i]);
+        const P u0 = P(a.u[i]), v0 = P(a.v[i]);
+        const P re = P(a.re[i]), im = P(a.im[i]);
+        const P sw = sqrt(P(a.w[i]));
 #pragma unroll
         for (int k = 0; k < KW; ++k) {
             const int slot = s0 + k;
@@ -140,11 +183,13 @@ __global__ __launch_bounds__(256) void k_sample_multi_b(SampleArgs<T> a, BatchPt
                 SampleArgs<T> ak = a;
                 walker_sample_args(ak, b, slot, b.w0 + slot);
                 bool oob = false;
-                const cpx<T> V = sample_point_sym(ak, u0, v0, oob);
+                cpx<P> V;
+                if constexpr (MIX) V = sample_point_sym2<T, double>(ak, walker_scalars_f64(a, b, b.w0 + slot), u0, v0, oob);
+                else V = sample_point_sym(ak, u0, v0, oob);
                 oob_any |= oob;
-                const T cr = (V.re - re) * sw;
-                const T ci = (V.im - im) * sw;
-                if constexpr (std::is_same<T, double>::value) acc[k] = cr * cr + ci * ci;
+                const P cr = (V.re - re) * sw;
+                const P ci = (V.im - im) * sw;
+                if constexpr (std::is_same<P, double>::value) acc[k] = cr * cr + ci * ci;
                 else acc[k] = double(cr) * cr + double(ci) * ci;
             }
         }
@@ -164,13 +209,14 @@ __global__ __launch_bounds__(256) void k_sample_multi_b(SampleArgs<T> a, BatchPt
     }
 }
 
-template <typename T>
+template <typename T, bool MIX = false>
 __global__ __launch_bounds__(256) void k_sample_grad_sym_b(SampleArgs<T> a, BatchPtrs<T> b) {
     const int slot = blockIdx.y, w = b.w0 + slot;
     walker_sample_args(a, b, slot, w);
     double* hh = b.Hh + slot * b.sHh;
     const GradArgs<T> g{hh, hh + b.hot_off, b.part + size_t(w) * 4 * b.ldPart + b.partOff, b.ldPart};
-    sample_grad_sym_body<T>(a, g, blockIdx.x);
+    if constexpr (MIX) sample_grad_sym_body<T, double>(a, walker_scalars_f64(a, b, w), g, blockIdx.x);
+    else sample_grad_sym_body<T, T>(a, a, g, blockIdx.x);
 }
 
 template <typename T, int LOG2N>

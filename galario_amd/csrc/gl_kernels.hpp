@@ -488,6 +488,7 @@ struct SampleArgs {
     double* partials;            // MODE_CHI2: one per workgroup
     unsigned* flag;              // bit 0: a point fell outside the grid
     double rho2;                 // > 0: squared uv extent in cells; points beyond it count as outside
+    double duv_d;                // duv in double (the FP32-bulk form's FP64 per-point arithmetic)
 };
 
 // Grid coordinates of a (rotated) point; oob when it is not inside the grid
@@ -569,24 +570,35 @@ __device__ __forceinline__ cpx<T> sample_point(const SampleArgs<T>& a, T u0, T v
 // Same on the real, even spectrum H: the corners are real, Z is real, the
 // phase of galario's amplitude/phase interpolation is 0 or pi, so
 // V = (copysign(A, Z), 0) before the phase shift.
-template <typename T>
-__device__ __forceinline__ cpx<T> sample_point_sym(const SampleArgs<T>& a, T u0, T v0, bool& oob) {
-    const GridPos<T> g = grid_pos(a, u0, v0);
+//
+// Storage and arithmetic may differ: the spectrum H is read from `a` (type T)
+// while the position, interpolation and phase use the scalars of `sc` and
+// arithmetic in P.  P = T (sc = a) is galario's arithmetic in that precision;
+// T = float, P = double is the FP32-bulk form with FP64 per-point arithmetic.
+template <typename T, typename P>
+__device__ __forceinline__ cpx<P> sample_point_sym2(const SampleArgs<T>& a, const SampleArgs<P>& sc, P u0, P v0,
+                                                    bool& oob) {
+    const GridPos<P> g = grid_pos(sc, u0, v0);
     oob = g.oob;
-    const T q = g.q, t = g.t;
+    const P q = g.q, t = g.t;
     const int half = a.nrow / 2;
     const int m0 = abs(g.fl_v - half), m1 = abs(g.fv1 - half);
-    const T y0 = a.H[h_index(g.fl_u, m0, a.ldH)], y1 = a.H[h_index(g.fl_u, m1, a.ldH)];
-    const T y3 = a.H[h_index(g.fu1, m0, a.ldH)], y2 = a.H[h_index(g.fu1, m1, a.ldH)];
-    const T z = (t * (q * ((y0 + y2) - (y1 + y3))) + (t * (y1 - y0) + q * (y3 - y0))) + y0;
-    const T a0 = fabs(y0), a1 = fabs(y1), a2 = fabs(y2), a3 = fabs(y3);
-    T amp = a0;
+    const P y0 = P(a.H[h_index(g.fl_u, m0, a.ldH)]), y1 = P(a.H[h_index(g.fl_u, m1, a.ldH)]);
+    const P y3 = P(a.H[h_index(g.fu1, m0, a.ldH)]), y2 = P(a.H[h_index(g.fu1, m1, a.ldH)]);
+    const P z = (t * (q * ((y0 + y2) - (y1 + y3))) + (t * (y1 - y0) + q * (y3 - y0))) + y0;
+    const P a0 = fabs(y0), a1 = fabs(y1), a2 = fabs(y2), a3 = fabs(y3);
+    P amp = a0;
     amp += (a3 - a0) * q;
     amp += (a1 - a0) * t;
     amp += (((a0 - a1) + a2) - a3) * t * q;
-    cpx<T> V = apply_phase_pt(a, g, cpx<T>{copysign(amp, z), T(0)});
-    if (oob) V = cpx<T>{T(NAN), T(NAN)};
+    cpx<P> V = apply_phase_pt(sc, g, cpx<P>{copysign(amp, z), P(0)});
+    if (oob) V = cpx<P>{P(NAN), P(NAN)};
     return V;
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> sample_point_sym(const SampleArgs<T>& a, T u0, T v0, bool& oob) {
+    return sample_point_sym2<T, T>(a, a, u0, v0, oob);
 }
 
 // Deterministic sum over a workgroup of any size <= 1024 (red: 16 doubles);

@@ -152,3 +152,62 @@ def walker_ensemble(rng, nwalkers: int, nr: int, Rmin: float, dR: float, inc: fl
     geom = np.stack([inc + 0.05 * z[:, 0], PA + 0.2 * z[:, 1], dRA + 0.005 * ARCSEC * z[:, 2],
                      dDec + 0.005 * ARCSEC * z[:, 3]], axis=1)
     return I, geom
+
+
+@dataclass
+class BatchWorkload:
+    """Many parameter sets over several datasets (float64 arrays).
+
+    ``sets``: one dict per dataset with nxy, dxy, u, v, re, im, w;
+    ``intensity`` [W, nr] and ``geom`` [W, 4] = (inc, PA, dRA, dDec)."""
+    nr: int
+    Rmin: float
+    dR: float
+    sets: list
+    intensity: np.ndarray
+    geom: np.ndarray
+
+
+def _batch_sets(rng, model, base: ProfileWorkload, shapes, bmin, bmax):
+    sets = []
+    for nxy, dxy, nvis in shapes:
+        base.nxy, base.dxy = nxy, dxy
+        base.u, base.v = log_uniform_uv(rng, nvis, bmin, bmax)
+        re, im, w = noisy_observation(rng, model(base))
+        sets.append(dict(nxy=nxy, dxy=dxy, u=base.u, v=base.v, re=re, im=im, w=w))
+    return sets
+
+
+def walkers_uneven_grids_workload(model, nwalkers: int = 256, seed: int = 3) -> BatchWorkload:
+    """Batched forward workload: nwalkers parameter sets (Nr = 300 bins of the ring+envelope
+    shape, Rmin = 1e-4", dR = 2"/300, walker_ensemble jitter) against three datasets of uneven
+    size, (nxy, Nvis) = (512, 1e4), (1024, 2e5), (2048, 1e6), all with dxy = 0.03" and
+    baselines log-uniform in [3e4, 3e6] wavelengths.
+
+    Draws (seed): per dataset in that order its uv then its noise (the model of each dataset
+    is evaluated at the base parameters inc = 40 deg, PA = 120 deg, dRA = dDec = 0.02"),
+    then the walker ensemble."""
+    rng = np.random.default_rng(seed)
+    nr, Rmin, dR, dxy = 300, 1e-4 * ARCSEC, 2.0 * ARCSEC / 300, 0.03 * ARCSEC
+    base = ProfileWorkload(intensity=ring_envelope_profile(nr, Rmin, dR), Rmin=Rmin, dR=dR, nxy=0, dxy=dxy,
+                           inc=40 * DEG, PA=120 * DEG, dRA=0.02 * ARCSEC, dDec=0.02 * ARCSEC, u=None, v=None)
+    sets = _batch_sets(rng, model, base, ((512, dxy, 10_000), (1024, dxy, 200_000), (2048, dxy, 1_000_000)),
+                       3e4, 3e6)
+    I, geom = walker_ensemble(rng, nwalkers, nr, Rmin, dR, base.inc, base.PA, base.dRA, base.dDec)
+    return BatchWorkload(nr=nr, Rmin=Rmin, dR=dR, sets=sets, intensity=I, geom=geom)
+
+
+def sets_three_datasets_4096_workload(model, nsets: int = 64, seed: int = 4) -> BatchWorkload:
+    """Batched forward+gradient workload: nsets parameter sets (Nr = 500 bins of the ring+envelope
+    shape, Rmin = 1e-4", dR = 0.004") against three datasets on one nxy = 4096, dxy = 0.01" grid
+    with Nvis = 5e5, 1.5e6 and 3e6 (5e6 in total), baselines log-uniform in [5e3, 1e7] wavelengths.
+
+    Draws (seed): as walkers_uneven_grids_workload."""
+    rng = np.random.default_rng(seed)
+    nr, Rmin, dR, dxy = 500, 1e-4 * ARCSEC, 0.004 * ARCSEC, 0.01 * ARCSEC
+    base = ProfileWorkload(intensity=ring_envelope_profile(nr, Rmin, dR), Rmin=Rmin, dR=dR, nxy=4096, dxy=dxy,
+                           inc=40 * DEG, PA=120 * DEG, dRA=0.02 * ARCSEC, dDec=0.02 * ARCSEC, u=None, v=None)
+    sets = _batch_sets(rng, model, base, ((4096, dxy, 500_000), (4096, dxy, 1_500_000), (4096, dxy, 3_000_000)),
+                       5e3, 1e7)
+    I, geom = walker_ensemble(rng, nsets, nr, Rmin, dR, base.inc, base.PA, base.dRA, base.dDec)
+    return BatchWorkload(nr=nr, Rmin=Rmin, dR=dR, sets=sets, intensity=I, geom=geom)

@@ -24,7 +24,7 @@ import torch
 
 import weakref
 
-from ._lib import BATCH_EAGER, DatasetF32, DatasetF64, check as _check, lib
+from ._lib import BATCH_EAGER, BATCH_MODE_F64_PHASE, BATCH_MODE_GRAD, DatasetF32, DatasetF64, check as _check, lib
 
 _SUF = {torch.float64: 'f64', torch.float32: 'f32'}
 
@@ -349,9 +349,15 @@ class BatchedChi2Profile:
     per-dataset values [W, S].  The first call for a given W times chunk
     sizes (synchronising) and captures the launch sequence in a HIP graph;
     later calls replay it on torch's current stream.  Points outside a grid
-    make the walker's chi2 NaN (``take_oob_flag``)."""
+    make the walker's chi2 NaN (``take_oob_flag``).
 
-    def __init__(self, sets, nr, Rmin, dR, max_walkers, check_central_pixel=True):
+    ``phase_f64=True`` (float32 datasets only) is the FP32-bulk form with FP64
+    geometry: profiles, spectra and dataset streams stay float32 while
+    ``geom`` is float64 and the rotation, grid position, interpolation, phase
+    argument and its sincos, residual and chi2 run in double on the float
+    values (the step galario performs in dreal, src/galario.cpp:877-884)."""
+
+    def __init__(self, sets, nr, Rmin, dR, max_walkers, check_central_pixel=True, phase_f64=False):
         sets = list(sets)
         if not sets:
             raise ValueError('at least one dataset is needed')
@@ -359,6 +365,10 @@ class BatchedChi2Profile:
         self.dtype, self.device = ds0.dtype, ds0.device
         self.nr, self.nsets, self.max_walkers = int(nr), len(sets), int(max_walkers)
         suf = _SUF[self.dtype]
+        self.phase_f64 = bool(phase_f64)
+        if self.phase_f64 and suf != 'f32':
+            raise ValueError('phase_f64 is the float32 datasets\' mixed form')
+        self.geom_dtype = torch.float64 if self.phase_f64 else self.dtype
         St = DatasetF64 if suf == 'f64' else DatasetF32
         arr = (St * len(sets))()
         self._keep = []
@@ -376,12 +386,17 @@ class BatchedChi2Profile:
                                                                int(bool(check_central_pixel)), ctypes.byref(h)))
         self._h = h
         self._fin = weakref.finalize(self, _destroy_plan, h.value)
-        self._run = getattr(lib, f'galario_batch_chi2_profile_{suf}')
-        self._run_grad = getattr(lib, f'galario_batch_chi2_profile_grad_{suf}')
+        x = 'x' if self.phase_f64 else ''
+        self._run = getattr(lib, f'galario_batch_chi2_profile_{suf}{x}')
+        self._run_grad = getattr(lib, f'galario_batch_chi2_profile_grad_{suf}{x}')
+
+    def _mode(self, grad):
+        return (BATCH_MODE_GRAD if grad else 0) | (BATCH_MODE_F64_PHASE if self.phase_f64 else 0)
 
     def _inputs(self, intensity, geom):
-        _check_tensors(self.dtype, intensity, geom)
-        if intensity.device != self.device:
+        _check_tensors(self.dtype, intensity)
+        _check_tensors(self.geom_dtype, geom)
+        if intensity.device != self.device or geom.device != self.device:
             raise ValueError('parameters must be on the datasets\' device')
         if intensity.dim() != 2 or intensity.shape[1] != self.nr:
             raise ValueError(f'intensity must be [W, {self.nr}]')
@@ -417,13 +432,13 @@ class BatchedChi2Profile:
     def info(self, nwalkers, grad=False):
         """(chunk size, walkers per sample workgroup) picked for nwalkers, and whether its graph is captured."""
         c, k, g = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
-        _check(lib.galario_batch_info(self._h, int(nwalkers), int(bool(grad)), ctypes.byref(c), ctypes.byref(k),
+        _check(lib.galario_batch_info(self._h, int(nwalkers), self._mode(grad), ctypes.byref(c), ctypes.byref(k),
                                       ctypes.byref(g)))
         return c.value, k.value, bool(g.value)
 
     def set_chunk(self, nwalkers, chunk, walkers_per_sample=4, grad=False):
         """Override the timed launch shape for nwalkers."""
-        _check(lib.galario_batch_set_chunk(self._h, int(nwalkers), int(bool(grad)), int(chunk),
+        _check(lib.galario_batch_set_chunk(self._h, int(nwalkers), self._mode(grad), int(chunk),
                                            int(walkers_per_sample)))
 
     def take_oob_flag(self) -> bool:
@@ -440,7 +455,7 @@ class _BatchedChi2Function(torch.autograd.Function):
     @staticmethod
     def forward(ctx, plan, intensity, geom):
         I = intensity.detach().contiguous()
-        g = geom.detach().to(I.device, I.dtype).contiguous()
+        g = geom.detach().to(I.device, plan.geom_dtype).contiguous()
         if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
             chi2, _, gI, gg = plan.grad(I, g)
             ctx.save_for_backward(gI, gg)

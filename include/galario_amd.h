@@ -248,7 +248,9 @@ int galario_chi2_profile_multi_dev_f32(void *stream, int nr, const float *intens
  * two from 128 to 8192 in double, 16384 in float) and cell size dxy, duv
  * (galario passes 1/(nxy*dxy), python/libcommon.pyx:723), and uvmax as for
  * the device-resident forms (<= 0: none; a grid's spectrum is cut only when
- * every dataset on it gives one). */
+ * every dataset on it gives one).  The float form takes duv in double: its
+ * FP32 kernels round it to float, the FP32-bulk form with FP64 geometry
+ * (galario_batch_chi2_profile_f32x) uses it as given. */
 typedef struct {
     int nxy;
     double dxy, duv;
@@ -258,7 +260,8 @@ typedef struct {
 } galario_dataset_f64;
 typedef struct {
     int nxy;
-    float dxy, duv;
+    float dxy;
+    double duv;
     int nd;
     const float *u, *v, *vis_obs_re, *vis_obs_im, *weights;
     double uvmax;
@@ -302,15 +305,34 @@ int galario_batch_chi2_profile_grad_f64(void *plan, void *stream, int nwalkers, 
 int galario_batch_chi2_profile_grad_f32(void *plan, void *stream, int nwalkers, const float *intensity,
                                         const float *geom, double *chi2_sets, double *chi2, double *grad_I,
                                         double *grad_geom, int flags);
-/* The launch shape picked for (nwalkers, grad): chunk size (0 before the
+
+/* FP32 bulk with FP64 geometry and phase, on a float plan: the profiles,
+ * row and half spectra and the dataset streams stay float (half the bytes of
+ * the double form), while geom [nwalkers][4] is double and every per-point
+ * step after the spectrum -- rotation, grid position, interpolation, the
+ * phase argument urot*2pi*dRArot + vrot*2pi*dDecrot and its sincos (the step
+ * galario performs in dreal, src/galario.cpp:877-884,893-894), the residual
+ * and chi2 -- runs in double on the float values; chi2 and the gradients
+ * accumulate in double.  Same outputs as the calls above. */
+int galario_batch_chi2_profile_f32x(void *plan, void *stream, int nwalkers, const float *intensity,
+                                    const double *geom, double *chi2_sets, double *chi2, int flags);
+int galario_batch_chi2_profile_grad_f32x(void *plan, void *stream, int nwalkers, const float *intensity,
+                                         const double *geom, double *chi2_sets, double *chi2, double *grad_I,
+                                         double *grad_geom, int flags);
+
+/* Launch modes of a plan (the `mode` of galario_batch_info / _set_chunk):
+ * each mode has its own timed launch shape and captured graph. */
+#define GALARIO_BATCH_MODE_GRAD 1       /* the _grad calls */
+#define GALARIO_BATCH_MODE_F64_PHASE 2  /* the _f32x calls (float plans) */
+/* The launch shape picked for (nwalkers, mode): chunk size (0 before the
  * first call) and walkers per sample workgroup (the forward sample reads each
  * dataset point once for that many walkers; 1 with the gradient), and
  * whether its graph is captured.  Any pointer may be null. */
-int galario_batch_info(void *plan, int nwalkers, int grad, int *chunk, int *walkers_per_sample,
+int galario_batch_info(void *plan, int nwalkers, int mode, int *chunk, int *walkers_per_sample,
                        int *graph_ready);
-/* Overrides the timed choice for (nwalkers, grad): chunk size and walkers
+/* Overrides the timed choice for (nwalkers, mode): chunk size and walkers
  * per sample workgroup (1, 2, 4 or 8). */
-int galario_batch_set_chunk(void *plan, int nwalkers, int grad, int chunk, int walkers_per_sample);
+int galario_batch_set_chunk(void *plan, int nwalkers, int mode, int chunk, int walkers_per_sample);
 /* Synchronises the plan's last stream; returns (and clears) its out-of-grid flag. */
 int galario_batch_take_oob_flag(void *plan, int *out);
 

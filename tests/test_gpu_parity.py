@@ -258,7 +258,7 @@ def test_profile_workload_single_4096():
     def model(wl):
         return O.sample_profile(wl.intensity, wl.Rmin, wl.dR, wl.dxy, wl.nxy, wl.inc, wl.dRA, wl.dDec, wl.duv,
                                 wl.PA, wl.u, wl.v, check_central=False)
-    wl = syn.ring_profile_workload(model, nvis=200_000, nxy=4096, seed=1)
+    wl = syn.ring_profile_workload(model, nvis=1_000_000, nxy=4096, seed=1)
     f = lambda a: np.ascontiguousarray(a, dtype=dt)  # noqa: E731
     I, u, v, re, im, w = map(f, (wl.intensity, wl.u, wl.v, wl.re, wl.im, wl.w))
     sc = {k: dt(x) for k, x in wl.kwargs().items()}
