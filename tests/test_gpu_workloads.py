@@ -11,12 +11,14 @@
     points whose interpolated spectrum changes sign between the two
     evaluations are dropped, for PA also points whose grid cell changes; the
     profile direction is smooth; the step is 1e-6 relative except for inc
-    (1e-8: inc moves every pixel radius across the profile's interpolation
-    knots, whose slope jumps give a central difference an O(h) error), and
-    it is shrunk tenfold while a pixel of the swept box crosses the last
-    profile bin between the two evaluations (galario's sweep drops pixels
-    beyond it, src/galario.cpp:1063-1065, a jump that a difference quotient
-    turns into an O(1/h) error);
+    (1e-8).  inc moves every pixel radius: a pixel crossing a knot of the
+    profile's linear interpolation (the jittered profiles have large slope
+    jumps there) gives a central difference an O(h) error, and one crossing
+    the last bin (galario's sweep drops pixels beyond it,
+    src/galario.cpp:1063-1065) an O(1/h) error.  So the point's inclination
+    is nudged (by at most 4e-7 relative) to a value where no pixel of the
+    swept box changes its profile bin between inc - h and inc + h, and every
+    difference and the analytic gradient are taken there;
   - the float path against the double path at every point: chi2 1e-5, the
     profile gradient 1e-4 norm-wise, the geometry gradients 1e-2.
 * Batched forward: 256 walkers (Nr=300) x datasets (512, 1e4), (1024, 2e5),
@@ -120,24 +122,31 @@ def _cells(wl, PA):
     return np.stack([np.floor(np.abs(urot) / wl.duv), np.floor(wl.nxy // 2 + s * vrot / wl.duv)])
 
 
-def _box_support(wl, inc):
-    """Pixels of the swept box below the last profile bin (galario.cpp:1063-1065) at inclination inc."""
-    nr = len(wl.intensity)
-    rmax = O.sweep_rmax(nr, wl.Rmin, wl.dR, wl.nxy, wl.dxy)
+def _box_bins(wl, inc):
+    """Profile bin of every pixel of the swept box at inclination inc (galario.cpp:1048-1071: r from
+    x / cos(inc) and y, bin floor((r - Rmin) / dR) clamped at 0)."""
+    rmax = O.sweep_rmax(len(wl.intensity), wl.Rmin, wl.dR, wl.nxy, wl.dxy)
     c = np.arange(-rmax + 1, rmax + 1) * wl.dxy
     r = np.sqrt((c[None, :] / np.cos(inc)) ** 2 + c[:, None] ** 2)
-    return np.maximum(np.floor((r - wl.Rmin) / wl.dR), 0) <= nr - 2
+    return np.maximum(np.floor((r - wl.Rmin) / wl.dR), 0)
 
 
 def _fd_cases(wl, I, g):
+    """The difference cases at parameter point (I, g); g[0] (inc) is nudged in place first (see above)."""
+    h0 = 1e-8 * abs(g[0])
+    for k in range(40):
+        inc = g[0] * (1 + 1e-8 * k)
+        if np.array_equal(_box_bins(wl, inc + h0), _box_bins(wl, inc - h0)):
+            break
+    else:
+        raise AssertionError('no inclination within 4e-7 relative without a pixel crossing a profile knot')
+    g[0] = inc
     R = np.arange(len(I))
     dirI = I * (np.sin(R / 37.0) + 0.5 * np.cos(R / 91.0))
     h = 1e-6
     cases = [('profile', I + h * dirI, I - h * dirI, g, g, 2 * h)]
     for j in range(4):
         hj = (1e-8 if j == 0 else 1e-6) * abs(g[j])
-        while j == 0 and not np.array_equal(_box_support(wl, g[0] + hj), _box_support(wl, g[0] - hj)):
-            hj /= 10
         gp, gm = g.copy(), g.copy()
         gp[j] += hj
         gm[j] -= hj

@@ -318,3 +318,68 @@ def test_oracle_vs_golden_direct_dft():
     vis = O.sample_image(img, 1.0, float(g['dRA']), float(g['dDec']), duv, 0.0, g['u'], g['v'])
     ref = g['vis_re'] + 1j * g['vis_im']
     assert np.abs(vis - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------- the float32 branches
+# The oracle's single-precision paths (galario's float build: dreal = float, with the C++'s promotions to
+# double) are pinned against the independent float64 formulations above, at the float bar 1e-5.
+@pytest.mark.parametrize("Rmin, dR, nrad, nxy, dxy, inc, mode",
+                         [(1e-6, 0.001, 2000, 1024, 0.2, 20., 'Gauss'),
+                          (1e-6, 0.001, 2000, 2048, 0.2, 44.23, 'Cos-Gauss'),
+                          (1e-6, 0.001, 2000, 1024, 0.3, 20., 'Gauss')])
+def test_float32_sweep_vs_interp_meshgrid(Rmin, dR, nrad, nxy, dxy, inc, mode):
+    """Float32 sweep (box and central pixel) against np.interp on the float64 meshgrid: 1e-5 of the
+    largest pixel; a dropped central pixel exceeds that bound."""
+    Rmin, dR, dxy, inc = Rmin * arcsec, dR * arcsec, dxy * arcsec, np.radians(inc)
+    I = radial_profile(Rmin, dR, nrad, mode, gauss_width=dxy * 6)
+    ref = sweep_interp(I, Rmin, dR, nxy, dxy, inc)
+    img = O.sweep(I.astype(np.float32), Rmin, dR, nxy, dxy, inc, dtype=np.float32)
+    assert img.dtype == np.float32
+    scale = np.abs(ref).max()
+    assert np.abs(img - ref).max() <= 1e-5 * scale
+    c = nxy // 2
+    assert abs(img[c, c] - ref[c, c]) <= 1e-5 * abs(ref[c, c])
+    bad = img.copy()
+    bad[c, c] = 0
+    assert np.abs(bad - ref).max() > 1e-5 * scale
+
+
+@pytest.mark.parametrize("Rmin, dR, dxy", [(1e-4, 0.003, 0.05), (1e-6, 0.001, 0.2), (1e-4, 2e-5, 0.01)])
+def test_float32_central_pixel_vs_trapezoid(Rmin, dR, dxy):
+    """The float32 central-pixel branch (its own promotions to double) against the float64 trapezoid
+    closed form of a smooth profile: 1e-5 relative."""
+    Rmin, dR, dxy = Rmin * arcsec, dR * arcsec, dxy * arcsec
+    nr = int((dxy / 2 - Rmin) / dR) + 10
+    R = Rmin + dR * np.arange(nr)
+    I = np.exp(-(R / (dxy * 0.7)) ** 2) + 0.2
+    ref = dxy ** 2 * central_trapezoid(I, Rmin, dR, dxy)
+    got = O.central_pixel_value(I.astype(np.float32), Rmin, dR, dxy, dtype=np.float32)
+    assert isinstance(got, np.float32)
+    assert abs(float(got) - ref) <= 1e-5 * abs(ref), (float(got), ref)
+
+
+def test_float32_pipeline_vs_direct_dft_at_grid_nodes():
+    """The float32 pipeline (sweep, shift, FFT, interpolation, phase, chi2) at grid-node uv points against
+    the float64 direct Fourier sum of the float64 image and math.fsum: visibilities 1e-5 of max|V|,
+    chi2 1e-5 relative."""
+    rng = np.random.default_rng(21)
+    N, dxy = 128, 0.03 * arcsec
+    Rmin, dR, nr = 1e-4 * arcsec, 0.001 * arcsec, 2500
+    R = Rmin + dR * np.arange(nr)
+    I = 1e10 * (np.exp(-((R - 0.6 * arcsec) / (0.2 * arcsec)) ** 2) + 0.3 * np.exp(-R / (0.5 * arcsec)))
+    inc, dRA, dDec = 0.5, 0.02 * arcsec, -0.01 * arcsec
+    duv = 1 / (N * dxy)
+    u = rng.integers(-N // 2 + 2, N // 2 - 1, 300) * duv
+    v = rng.integers(-N // 2 + 2, N // 2 - 1, 300) * duv
+    img64 = O.sweep(I, Rmin, dR, N, dxy, inc, check_central=False)
+    ref = O.direct_dft_vis(img64, dxy, u, v, dRA, dDec, 0.0)
+    f = np.float32
+    vis = O.sample_profile(I.astype(f), f(Rmin), f(dR), f(dxy), N, f(inc), f(dRA), f(dDec), f(duv), f(0.0),
+                           u.astype(f), v.astype(f), dtype=f, check_central=False)
+    assert vis.dtype == np.complex64
+    assert np.abs(vis - ref).max() <= 1e-5 * np.abs(ref).max()
+    re, im = ref.real + rng.standard_normal(300) * 1e6, ref.imag + rng.standard_normal(300) * 1e6
+    w = rng.uniform(0.5, 2.0, 300) * 1e-12
+    c_ref = math.fsum(w * ((ref.real - re) ** 2 + (ref.imag - im) ** 2))
+    c32 = O.reduce_chi2(re.astype(f), im.astype(f), vis, w.astype(f), dtype=f)
+    assert abs(float(c32) - c_ref) <= 1e-5 * c_ref
