@@ -113,6 +113,8 @@ def _load():
             # the device-resident forms end with uvmax (double; <= 0: no uv extent)
             'chi2_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp, _vp,
                                  _vp, _c_int, _vp, _c_double],
+            'chi2_profile_noalloc_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp,
+                                         _vp, _vp, _c_int, _vp, _c_double],
             'chi2_profile_grad_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _vp,
                                       _vp, _vp, _c_int, _vp, _vp, _vp, _c_double],
             'sample_profile_dev': [_vp, _c_int, _vp, R, R, R, _c_int, R, R, R, R, R, _c_int, _vp, _vp, _c_int,

@@ -30,6 +30,13 @@
 
 using namespace gl;
 
+// Scratch of the allocation-free chi2 entry point (galario_chi2_profile_noalloc_dev_*): the row
+// spectra, the half spectrum and the per-workgroup chi2 partials of one evaluation.  One array per
+// device, held until the process ends.  Host code never passes this name to a kernel: its
+// per-device address comes from hipGetSymbolAddress at set-up (static_setup_current).
+constexpr size_t SCRATCH_BYTES = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
 namespace {
 
 // ------------------------------------------------------------------ errors
@@ -1565,6 +1572,103 @@ int batch_run(void* h, void* stream, int W, const T* I, const G* geom, double* c
 
 int g_threads = 16;
 
+// ====================================================================== allocation-free chi2
+// Set-up per device (galario_init, galario_use_gpu): the device address of `scratch` and every
+// twiddle table the allocation-free form can need, built (allocated) here so that its calls only
+// look them up.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_static_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+void static_setup_current() {
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) fail(GALARIO_ERR_RUNTIME, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (g_scratch_ptr[dev]) return;
+    void* p = nullptr;
+    HC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(GALARIO_ERR_RUNTIME, "Error in hip: no device address for the scratch array");
+    for (int L = 7; L <= max_fast_log2<double>(); ++L) (void)twiddles<double>(1 << L);
+    for (int L = 7; L <= max_fast_log2<float>(); ++L) (void)twiddles<float>(1 << L);
+    g_scratch_ptr[dev] = p;
+}
+
+// Twiddle table built at set-up (never allocates).
+template <typename T>
+const cpx<T>* twiddles_built(int dev, int N) {
+    std::lock_guard<std::mutex> lock(g_tw_mu);
+    auto it = g_tw.find(std::make_tuple(dev, N, int(sizeof(T))));
+    if (it == g_tw.end()) fail(GALARIO_ERR_RUNTIME, "internal: twiddle table missing after set-up");
+    return static_cast<const cpx<T>*>(it->second);
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// chi2_profile_dev with its scratch in `scratch`: the same checks and kernels (chi2 is bit-identical),
+// no allocation and no library stream or event; points outside the grid make chi2 NaN.
+template <typename T>
+int chi2_profile_noalloc_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy, T inc, T dRA, T dDec,
+                             T duv, T PA, int nd, const T* u, const T* v, const T* re, const T* im, const T* wt,
+                             int check_central, double* chi2_d, double uvmax) {
+    return guard([&] {
+        check_nd(nd);
+        ProfileSetup<T> ps = setup_profile<T>(nr, Rmin, dR, dxy, nxy, inc, check_central);
+        SampleSetup<T> ss = setup_sample<T>(PA, dRA, dDec);
+        check_dev_inputs(I, nd, u, v, true, re, im, wt);
+        if (!chi2_d) invalid("check input", "null chi2 pointer");
+        const int L = fast_log2<T>(nxy);
+        if (L < 0)
+            invalid("chi2 without allocation", "nxy = " + std::to_string(nxy) + " (expect a power of two from 128 to " +
+                                                   std::to_string(1 << max_fast_log2<T>()) + ")");
+        const SymGeom g = sym_geom(L, nxy, ps.rmax);
+        const size_t nb = blocks_for(size_t(nd), 256);
+        const size_t o_Ts = align256(sizeof(double) * std::max<size_t>(nb, 1));
+        const size_t o_H = align256(o_Ts + sizeof(T) * size_t(2 * g.pairs_r) * size_t(g.ldT));
+        const size_t need = o_H + sizeof(T) * size_t(g.C + 1) * size_t(g.ldH);
+        if (need > SCRATCH_BYTES)
+            invalid("chi2 without allocation",
+                    "this evaluation needs " + std::to_string(need) + " bytes of scratch (set by nxy, the swept radius "
+                    "and the number of points); the library holds " + std::to_string(SCRATCH_BYTES));
+        int dev = 0;
+        HC(hipGetDevice(&dev));
+        char* base = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(g_static_mu);
+            if (dev >= 0 && dev < MAX_DEVICES) base = static_cast<char*>(g_scratch_ptr[dev]);
+        }
+        if (!base)
+            invalid("chi2 without allocation", "device " + std::to_string(dev) +
+                                                   " is not set up: call galario_init() (or galario_use_gpu()) "
+                                                   "with it current first");
+        const cpx<T>* tw = twiddles_built<T>(dev, nxy);
+        double* part = reinterpret_cast<double*>(base);
+        T* Ts = reinterpret_cast<T*>(base + o_Ts);
+        T* H = reinterpret_cast<T*>(base + o_H);
+        const Extent ex = uv_extent<T>(uvmax, double(duv));
+        ps.sw.I = I;
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const SymRowArgs<T> ra{ps.sw, nxy, g.Rs, g.nrows, g.ldT, tw, Ts, nullptr, 0, nullptr};
+        const SymColArgs<T> ca{Ts, nxy, g.Rs, g.ldT, tw, H, g.ldH, ex.rc2};
+        launch_rowpass_sym<T>(L, g.pairs_r, s, ra);
+        launch_colpass_sym<T>(L, nxy / 4, s, ca);
+        SampleArgs<T> sa = sample_args<T>(nullptr, 0, 0, nxy, g.C, T(1), duv, ss, nd, u, v);
+        sa.H = H;
+        sa.ldH = g.ldH;
+        sa.rho2 = ex.rho2;
+        sa.re = re;
+        sa.im = im;
+        sa.w = wt;
+        sa.partials = part;
+        sa.flag = nullptr;
+        const unsigned ng = unsigned((nb + SAMPLE_PPT - 1) / SAMPLE_PPT);
+        if (nb) hipLaunchKernelGGL((k_sample_chi2_n<T, true, SAMPLE_PPT>), dim3(ng), dim3(256), 0, s, sa, int(nb));
+        hipLaunchKernelGGL((k_sum_partials<T>), dim3(1), dim3(1024), 0, s, part, int(nb), chi2_d,
+                           static_cast<T*>(nullptr));
+        HC(hipGetLastError());
+    });
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -1573,7 +1677,14 @@ extern "C" {
 const char* galario_last_error(void) { return t_err.c_str(); }
 
 int galario_init(void) {
-    return guard([] {});
+    return guard([] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {   // no GPU visible: nothing to set up
+            (void)hipGetLastError();
+            return;
+        }
+        static_setup_current();
+    });
 }
 
 int galario_cleanup(void) {
@@ -1612,7 +1723,10 @@ int galario_ngpus(int* out) {
 }
 
 int galario_use_gpu(int device_id) {
-    return guard([&] { HC(hipSetDevice(device_id)); });
+    return guard([&] {
+        HC(hipSetDevice(device_id));
+        static_setup_current();
+    });
 }
 
 void galario_free(void* ptr) { std::free(ptr); }
@@ -1688,6 +1802,17 @@ GL_BOTH(GL_MAIN)
     }
 
 GL_BOTH(GL_GRAD)
+
+#define GL_NOALLOC(SUF, T)                                                                                        \
+    int galario_chi2_profile_noalloc_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy,    \
+                                               T inc, T dRA, T dDec, T duv, T PA, int nd, const T* u, const T* v, \
+                                               const T* re, const T* im, const T* w, int cc, double* chi2_d,      \
+                                               double uvmax) {                                                    \
+        return chi2_profile_noalloc_dev<T>(stream, nr, I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, nd, u, v,  \
+                                           re, im, w, cc, chi2_d, uvmax);                                         \
+    }
+
+GL_BOTH(GL_NOALLOC)
 
 #define GL_MULTI(SUF, T)                                                                                          \
     int galario_sample_profile_multi_dev_##SUF(void* stream, int nr, const T* I, T Rmin, T dR, T dxy, int nxy,    \

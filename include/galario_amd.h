@@ -43,7 +43,9 @@ const char *galario_last_error(void);
 
 /* ---- management: src/galario.h:51-58 ---------------------------------- */
 
-/* replaces galario::init() (src/galario.h:52; python/__init_module__.py.in:22) */
+/* replaces galario::init() (src/galario.h:52; python/__init_module__.py.in:22):
+ * with a GPU visible, sets up the current device for the allocation-free
+ * entry point (galario_chi2_profile_noalloc_dev_*); without one, does nothing */
 int galario_init(void);
 /* replaces galario::cleanup() (src/galario.h:53): frees the calling thread's
  * workspaces.  The FFT twiddle tables (shared by all threads and by captured
@@ -57,7 +59,7 @@ int galario_threads(int num);
 /* replaces galario::ngpus() (src/galario.h:57) */
 int galario_ngpus(int *out);
 /* replaces galario::use_gpu(int) (src/galario.h:58): selects the device of
- * the calling thread */
+ * the calling thread and sets it up as galario_init does */
 int galario_use_gpu(int device_id);
 /* replaces galario::galario_free(void*) (src/galario.h:42) for buffers from
  * galario_copy_input_* */
@@ -213,6 +215,36 @@ int galario_chi2_profile_grad_dev_f32(void *stream, int nr, const float *intensi
                                       const float *vis_obs_re, const float *vis_obs_im, const float *weights,
                                       int check_central, double *chi2_d, double *grad_I, double *grad_geom,
                                       double uvmax);
+
+/* chi2 of a profile on device arrays without allocating device memory.  No
+ * counterpart in galario: for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks and kernels as galario_chi2_profile_dev_* (chi2 is
+ * bit-identical); the kernels' scratch -- the row spectra, the half spectrum
+ * and the per-workgroup partials -- lives in one 48 MiB device array per GPU,
+ * which the library holds from the first use of that GPU until the process
+ * ends.  Limits (else GALARIO_ERR_INVALID, nothing enqueued): nxy a power of
+ * two from 128 to 8192 (double) or 16384 (float), and
+ *   8 ceil(nd/256) + s (2 ceil((Rs'+1)/2) ldT + (nxy/2+2) ldH) <= 48 MiB
+ * (each term rounded up to 256 bytes; s = 8 or 4 bytes, Rs' = min(rmax+1,
+ * nxy/2), ldT = ldH = nxy/2+1 rounded up to 8): nxy = 4096 in double with a
+ * swept radius of 201 pixels and 1e6 points needs 37 MB; nxy = 8192 in double
+ * does not fit.  Set-up: galario_init() or galario_use_gpu() with the device
+ * current (they resolve the array's device address and build the FFT
+ * twiddle tables); a device not set up gives GALARIO_ERR_INVALID.  Points
+ * outside the grid make chi2 NaN (no flag is set).  Every call on one GPU
+ * uses the same array: calls that overlap on one GPU (on any streams) race,
+ * so order them. */
+int galario_chi2_profile_noalloc_dev_f64(void *stream, int nr, const double *intensity, double Rmin, double dR,
+                                         double dxy, int nxy, double inc, double dRA, double dDec, double duv,
+                                         double PA, int nd, const double *u, const double *v,
+                                         const double *vis_obs_re, const double *vis_obs_im,
+                                         const double *weights, int check_central, double *chi2_d, double uvmax);
+int galario_chi2_profile_noalloc_dev_f32(void *stream, int nr, const float *intensity, float Rmin, float dR,
+                                         float dxy, int nxy, float inc, float dRA, float dDec, float duv,
+                                         float PA, int nd, const float *u, const float *v, const float *vis_obs_re,
+                                         const float *vis_obs_im, const float *weights, int check_central,
+                                         double *chi2_d, double uvmax);
 
 /* Several sources with the same profile, inclination and PA at their own
  * offsets (galario's multi-source recipe, docs/tech-specs.rst:45-57: sum the
