@@ -9,7 +9,10 @@ A step is one chi2Profile evaluation with all inputs resident in HBM.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
 
 Prints one JSON line.  `value` is timed with HIP events on the kernels' stream
-around K calls (no per-stage events in that loop).  `roofline` is measured
+around exactly K calls (no per-stage events in that loop); `steady_state`
+repeats the measurement over 250 calls in 5 blocks and reports the median
+block.  Every rate carries its HBM-roofline fraction `frac` = algorithmic
+bytes per evaluation x evaluations/s / 8 TB/s.  `roofline` is measured
 live: each forward kernel is launched 20 times back to back inside a pair of
 HIP events (galario_stage_repeat), giving its average launch duration;
 `achieved` = algorithmic bytes / duration for the kernel that moves the most
@@ -35,30 +38,57 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "chi2Profile evals/s fwd & fwd+grad, nxy=4096 Nvis=1e6 FP64; % HBM roofline"
 
 
-def stage_bytes(nxy, nvis, nr, rmax, grad=False, s=8):
+def stage_bytes(nxy, nvis, nr, rmax, grad=False, s=8, kw=1):
     """Algorithmic HBM bytes per launch of each stage of the profile path (each
-    needed array touched once).  The swept image is even in x and y, so its
-    spectrum is real and even: the row pass writes Rs'+1 real rows
-    (Rs' = min(rmax+1, nxy/2)), the column pass reads them and writes the
-    (nxy/2+1) x (nxy/2+1) real half spectrum H, the sample kernel streams
-    u, v, Re, Im, w and gathers 4 real corners per point.  With the gradient
-    it also scatters 4 corner gradients (FP64 atomics) into the adjoint Hh;
-    the adjoint column pass reads Hh, clears it and writes Rs'+1 adjoint rows;
-    the adjoint row pass reads them (the profile gradient is a few KB)."""
+    needed array touched once), s bytes per stored real.  The swept image is
+    even in x and y, so its spectrum is real and even: the row pass writes
+    Rs'+1 real rows (Rs' = min(rmax+1, nxy/2)), the column pass reads them and
+    writes the (nxy/2+1) x (nxy/2+1) real half spectrum H, the sample kernel
+    streams u, v, Re, Im, w (once per kw parameter sets in the batched form)
+    and gathers 4 real corners per point.  With the gradient it also scatters
+    4 corner gradients (FP64 atomics) into the adjoint Hh (double in both
+    precisions); the adjoint column pass reads Hh, clears it and writes Rs'+1
+    adjoint rows; the adjoint row pass reads them (the profile gradient is a
+    few KB)."""
     C = nxy // 2 + 1
     rs = min(rmax + 1, nxy // 2)
     rows = rs + 1
     half = C * (nxy // 2 + 1) * s
+    half_adj = C * (nxy // 2 + 1) * 8
     b = {
         'row_pass': rows * C * s + nr * s,
         'column_pass': rows * C * s + half,
-        'sample': 5 * s * nvis + 4 * s * nvis,
+        'sample': int(5 * s * nvis / kw) + 4 * s * nvis,
     }
     if grad:
         b['sample'] += 4 * 8 * nvis
-        b['adjoint_column_pass'] = 2 * half + rows * C * s
+        b['adjoint_column_pass'] = 2 * half_adj + rows * C * s
         b['adjoint_row_pass'] = rows * C * s + 2 * nr * 8
     return b
+
+
+def swept_rmax(nxy, nr, Rmin, dR, dxy):
+    """Half side of the swept box: min(ceil((Rmin + nr dR) / dxy), nxy/2) (galario.cpp:1200)."""
+    import math
+    return min(int(math.ceil((Rmin + nr * dR) / dxy)), nxy // 2)
+
+
+def eval_bytes(grids, nr, grad=False, s=8, kw=1):
+    """Algorithmic bytes of one parameter set's evaluation over datasets grouped by grid: grids is a
+    list of (nxy, rmax, [nvis of each dataset on that grid]); one spectrum (and, with grad, one adjoint
+    spectrum) per grid, one sample per dataset."""
+    tot = 0
+    for nxy, rmax, nvis_list in grids:
+        b = stage_bytes(nxy, 0, nr, rmax, grad=grad, s=s, kw=kw)
+        tot += sum(v for k, v in b.items() if k != 'sample')
+        for nv in nvis_list:
+            tot += stage_bytes(nxy, nv, nr, rmax, grad=grad, s=s, kw=kw)['sample']
+    return tot
+
+
+def frac_of(nbytes, evals_per_s):
+    """HBM-roofline fraction: algorithmic bytes per evaluation x evaluations/s over 8 TB/s."""
+    return round(nbytes * evals_per_s / (HBM_PEAK_GBS * 1e9), 4)
 
 
 KERNEL_NAMES = {'row_pass': 'k_rowpass_sym<double, 12>', 'column_pass': 'k_colpass_sym<double, 12>',
@@ -114,8 +144,10 @@ def batched_rates(ta, ds, wl, I, chi2_plain, counts, counts_grad, steps, stream)
     Gt = torch.from_numpy(np.ascontiguousarray(gb)).to(dev)
     plan = ta.BatchedChi2Profile([(ds, wl.nxy, wl.dxy)], nr, wl.Rmin, wl.dR, max_walkers=pmax,
                                  check_central_pixel=False)
+    grids = [(wl.nxy, swept_rmax(wl.nxy, nr, wl.Rmin, wl.dR, wl.dxy), [len(wl.u)])]
     out = {'workload': f'batched form: P parameter sets on the same dataset (nxy={wl.nxy}, Nvis={len(wl.u)}), '
-                       f'one captured HIP graph per call', 'forward': {}, 'forward_grad': {}}
+                       f'one captured HIP graph per call; frac = algorithmic bytes per set (the streams counted '
+                       f'once per walkers_per_sample sets) x sets/s / 8 TB/s', 'forward': {}, 'forward_grad': {}}
 
     def rate(fn, P, reps):
         fn()                                                  # tuning + capture
@@ -140,15 +172,20 @@ def batched_rates(ta, ds, wl, I, chi2_plain, counts, counts_grad, steps, stream)
         torch.cuda.synchronize()
         rel = abs(float(c[0]) - chi2_plain) / abs(chi2_plain)
         chunk, kw, _ = plan.info(P)
-        out['forward'][str(P)] = {'evals_per_s': round(P * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4),
-                                  'chunk': chunk, 'walkers_per_sample': kw, 'chi2_rel_diff_vs_plain': rel}
+        ev = P * 1000.0 / ms
+        nb = eval_bytes(grids, nr, kw=kw)
+        out['forward'][str(P)] = {'evals_per_s': round(ev, 1), 'ms_per_batch': round(ms, 4),
+                                  'chunk': chunk, 'walkers_per_sample': kw, 'chi2_rel_diff_vs_plain': rel,
+                                  'alg_bytes_per_eval': nb, 'frac': frac_of(nb, ev)}
     for P in counts_grad:
         Ip, Gp = It[:P].contiguous(), Gt[:P].contiguous()
         reps = max(5, min(steps, int(1000 // P)))
         ms = rate(lambda: plan.grad(Ip, Gp), P, reps)
         chunk, _, _ = plan.info(P, grad=True)
-        out['forward_grad'][str(P)] = {'evals_per_s': round(P * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4),
-                                       'chunk': chunk}
+        ev = P * 1000.0 / ms
+        nb = eval_bytes(grids, nr, grad=True)
+        out['forward_grad'][str(P)] = {'evals_per_s': round(ev, 1), 'ms_per_batch': round(ms, 4),
+                                       'chunk': chunk, 'alg_bytes_per_eval': nb, 'frac': frac_of(nb, ev)}
     plan.close()
     return out
 
@@ -201,7 +238,10 @@ def extra_configs(ta, g, stream, steps):
 
         ms = timeit(one, max(16, min(steps, 64)))
         grads[name] = [one() for _ in range(16)]
-        res[name] = {'fwd_grad_evals_per_s': round(1000.0 / ms, 1), 'ms_per_eval': round(ms, 4)}
+        nb = eval_bytes([(wl.nxy, swept_rmax(wl.nxy, 500, wl.Rmin, wl.dR, wl.dxy), [len(wl.u)])], 500, grad=True,
+                        s=8 if dt == torch.float64 else 4)
+        res[name] = {'fwd_grad_evals_per_s': round(1000.0 / ms, 1), 'ms_per_eval': round(ms, 4),
+                     'alg_bytes_per_eval': nb, 'frac': frac_of(nb, 1000.0 / ms)}
         del ds
     rel = max(float(torch.linalg.norm(grads['f32'][k][1] - grads['f64'][k][1]) / torch.linalg.norm(grads['f64'][k][1]))
               for k in range(16))
@@ -209,67 +249,74 @@ def extra_configs(ta, g, stream, steps):
     out['differentiation_nxy2048_nvis2e6'] = res
 
     # batched forward: 256 walkers x datasets (512, 1e4), (1024, 2e5), (2048, 1e6); Nr=300
-    rng = np.random.default_rng(3)
-    Rmin, dR, dxy = 1e-4 * A, 2.0 * A / 300, 0.03 * A
-    base = syn.ProfileWorkload(intensity=syn.ring_envelope_profile(300, Rmin, dR), Rmin=Rmin, dR=dR, nxy=0,
-                               dxy=dxy, inc=40 * syn.DEG, PA=120 * syn.DEG, dRA=0.02 * A, dDec=0.02 * A, u=None, v=None)
-    sets = []
-    for nxy, nvis in ((512, 10_000), (1024, 200_000), (2048, 1_000_000)):
-        base.nxy = nxy
-        base.u, base.v = syn.log_uniform_uv(rng, nvis, 3e4, 3e6)
-        re, im, w = syn.noisy_observation(rng, model(base))
-        sets.append((ta.Dataset(base.u, base.v, re, im, w, device=dev, dtype=f64), nxy, dxy))
-    I256, G256 = syn.walker_ensemble(rng, 256, 300, Rmin, dR, base.inc, base.PA, base.dRA, base.dDec)
-    plan = ta.BatchedChi2Profile(sets, 300, Rmin, dR, max_walkers=256, check_central_pixel=False)
-    It, Gt = torch.from_numpy(I256).to(dev), torch.from_numpy(G256).to(dev)
+    bw = syn.walkers_uneven_grids_workload(model)
+    sets = [(ta.Dataset(d['u'], d['v'], d['re'], d['im'], d['w'], device=dev, dtype=f64), d['nxy'], d['dxy'])
+            for d in bw.sets]
+    plan = ta.BatchedChi2Profile(sets, bw.nr, bw.Rmin, bw.dR, max_walkers=256, check_central_pixel=False)
+    It, Gt = torch.from_numpy(bw.intensity).to(dev), torch.from_numpy(bw.geom).to(dev)
     ms = timeit(lambda: plan(It, Gt), 10)
     chunk, kw, _ = plan.info(256)
     per = plan(It, Gt, return_sets=True)[1][0].cpu().numpy()
-    ref = [ds.chi2_profile(It[0], Rmin, dR, nxy, dx, check_central_pixel=False, inc=float(G256[0, 0]),
-                           PA=float(G256[0, 1]), dRA=float(G256[0, 2]), dDec=float(G256[0, 3])).item()
+    G0 = bw.geom[0]
+    ref = [ds.chi2_profile(It[0], bw.Rmin, bw.dR, nxy, dx, check_central_pixel=False, inc=float(G0[0]),
+                           PA=float(G0[1]), dRA=float(G0[2]), dDec=float(G0[3])).item()
            for ds, nxy, dx in sets]
+    grids = [(d['nxy'], swept_rmax(d['nxy'], bw.nr, bw.Rmin, bw.dR, d['dxy']), [len(d['u'])]) for d in bw.sets]
+    nb = eval_bytes(grids, bw.nr, kw=kw)
+    ev = 256 * 1000.0 / ms
     out['batched_256_walkers_3_datasets_fwd'] = {
         'workload': '256 walkers (Nr=300) x datasets (nxy, Nvis) = (512, 1e4), (1024, 2e5), (2048, 1e6), FP64, '
                     'per-walker chi2 summed over the datasets',
-        'walker_evals_per_s': round(256 * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4), 'chunk': chunk,
-        'walkers_per_sample': kw,
+        'walker_evals_per_s': round(ev, 1), 'ms_per_batch': round(ms, 4), 'chunk': chunk,
+        'walkers_per_sample': kw, 'alg_bytes_per_walker_eval': nb, 'frac': frac_of(nb, ev),
         'chi2_rel_diff_vs_plain': float(np.max(np.abs(per - ref) / np.abs(ref)))}
     plan.close()
     del sets
 
-    # batched forward+gradient: 64 sets x 3 datasets at nxy=4096 (Nvis 5e5, 1.5e6, 3e6), Nr=500
-    rng = np.random.default_rng(4)
-    Rmin, dR, dxy, nxy = 1e-4 * A, 0.004 * A, 0.01 * A, 4096
-    base = syn.ProfileWorkload(intensity=syn.ring_envelope_profile(500, Rmin, dR), Rmin=Rmin, dR=dR, nxy=nxy,
-                               dxy=dxy, inc=40 * syn.DEG, PA=120 * syn.DEG, dRA=0.02 * A, dDec=0.02 * A, u=None, v=None)
-    raw = []
-    for nvis in (500_000, 1_500_000, 3_000_000):
-        base.u, base.v = syn.log_uniform_uv(rng, nvis, 5e3, 1e7)
-        raw.append((base.u, base.v) + tuple(syn.noisy_observation(rng, model(base))))
-    I64, G64 = syn.walker_ensemble(rng, 64, 500, Rmin, dR, base.inc, base.PA, base.dRA, base.dDec)
+    # batched forward+gradient: 64 sets x 3 datasets at nxy=4096 (Nvis 5e5, 1.5e6, 3e6), Nr=500: FP64, FP32
+    # bulk with FP64 geometry / phase / per-point arithmetic, and pure FP32, each against the FP64 plain path
+    bw = syn.sets_three_datasets_4096_workload(model)
     res = {'workload': '64 parameter sets (Nr=500) x 3 datasets at nxy=4096 (Nvis 5e5, 1.5e6, 3e6; 5e6 total), '
-                       'chi2 + gradient per set'}
-    ref = None
-    for name, dt in (('f64', torch.float64), ('f32', torch.float32)):
-        sets = [(ta.Dataset(*r, device=dev, dtype=dt), nxy, dxy) for r in raw]
-        plan = ta.BatchedChi2Profile(sets, 500, Rmin, dR, max_walkers=64, check_central_pixel=False)
-        It, Gt = torch.from_numpy(I64).to(dev, dt), torch.from_numpy(G64).to(dev, dt)
+                       'chi2 + gradient per set; errors of sets 0, 8, ..., 56 against the FP64 plain path'}
+    grids = [(4096, swept_rmax(4096, bw.nr, bw.Rmin, bw.dR, bw.sets[0]['dxy']), [len(d['u']) for d in bw.sets])]
+    sets64 = [(ta.Dataset(d['u'], d['v'], d['re'], d['im'], d['w'], device=dev, dtype=f64), d['nxy'], d['dxy'])
+              for d in bw.sets]
+    It64 = torch.from_numpy(bw.intensity).to(dev)
+    check = list(range(0, 64, 8))
+    refs = {}
+    for w in check:
+        G = bw.geom[w]
+        r = [ds.chi2_profile_grad(It64[w], bw.Rmin, bw.dR, nxy, dxy, check_central_pixel=False, inc=float(G[0]),
+                                  PA=float(G[1]), dRA=float(G[2]), dDec=float(G[3])) for ds, nxy, dxy in sets64]
+        refs[w] = (sum(x[0].item() for x in r), sum(x[1] for x in r), sum(x[2] for x in r))
+    for name, dt, mixed in (('f64', torch.float64, False), ('f32_bulk_f64_phase', torch.float32, True),
+                            ('f32', torch.float32, False)):
+        sets = sets64 if dt == f64 else [(ta.Dataset(d['u'], d['v'], d['re'], d['im'], d['w'], device=dev,
+                                                     dtype=dt), d['nxy'], d['dxy']) for d in bw.sets]
+        plan = ta.BatchedChi2Profile(sets, bw.nr, bw.Rmin, bw.dR, max_walkers=64, check_central_pixel=False,
+                                     phase_f64=mixed)
+        It = torch.from_numpy(bw.intensity).to(dev, dt)
+        Gt = torch.from_numpy(bw.geom).to(dev, f64 if mixed else dt)
         ms = timeit(lambda: plan.grad(It, Gt), 5)
         c, _, gI, gg = plan.grad(It, Gt)
         chunk, _, _ = plan.info(64, grad=True)
-        if ref is None:   # FP64 plain path, set 0
-            sets64 = sets
-            r = [ds.chi2_profile_grad(It[0], Rmin, dR, nxy, dxy, check_central_pixel=False, inc=float(G64[0, 0]),
-                                      PA=float(G64[0, 1]), dRA=float(G64[0, 2]), dDec=float(G64[0, 3]))
-                 for ds, _, _ in sets64]
-            ref = (sum(x[0].item() for x in r), sum(x[1] for x in r), sum(x[2] for x in r))
-        res[name] = {'fwd_grad_set_evals_per_s': round(64 * 1000.0 / ms, 1), 'ms_per_batch': round(ms, 4),
-                     'chunk': chunk,
-                     'chi2_rel_err_vs_f64_plain': abs(float(c[0]) - ref[0]) / abs(ref[0]),
-                     'grad_I_rel_err_vs_f64_plain': float(torch.linalg.norm(gI[0] - ref[1]) / torch.linalg.norm(ref[1])),
-                     'grad_geom_max_rel_err_vs_f64_plain': float(torch.max(torch.abs(gg[0] - ref[2]) / torch.abs(ref[2])))}
+        ev = 64 * 1000.0 / ms
+        nb = eval_bytes(grids, bw.nr, grad=True, s=8 if dt == f64 else 4)
+        e = [0.0, 0.0, 0.0]
+        for w in check:
+            c_ref, gI_ref, gg_ref = refs[w]
+            e = [max(e[0], abs(float(c[w]) - c_ref) / abs(c_ref)),
+                 max(e[1], float(torch.linalg.norm(gI[w] - gI_ref) / torch.linalg.norm(gI_ref))),
+                 max(e[2], float(torch.max(torch.abs(gg[w] - gg_ref) / torch.abs(gg_ref))))]
+        res[name] = {'fwd_grad_set_evals_per_s': round(ev, 1), 'ms_per_batch': round(ms, 4), 'chunk': chunk,
+                     'alg_bytes_per_set_eval': nb, 'frac': frac_of(nb, ev),
+                     'chi2_rel_err_vs_f64_plain': e[0], 'grad_I_rel_err_vs_f64_plain': e[1],
+                     'grad_geom_max_rel_err_vs_f64_plain': e[2]}
         plan.close()
-        del sets, plan
+        del plan
+        if dt != f64:
+            del sets
+    del sets64
     out['batched_64_sets_3_datasets_nxy4096_grad'] = res
 
     # multi-source summation: K sources sharing the profile, inc and PA, on the main workload's shape
@@ -284,7 +331,10 @@ def extra_configs(ta, g, stream, steps):
                            dtype=f64, device=dev)
         ms = timeit(lambda: ds.chi2_profile_multi(*args, src, PA=wl.PA, inc=wl.inc, check_central_pixel=False),
                     max(20, min(steps, 200)))
-        res[f'K{K}'] = {'evals_per_s': round(1000.0 / ms, 1), 'ms_per_eval': round(ms, 4)}
+        nb = eval_bytes([(wl.nxy, swept_rmax(wl.nxy, len(wl.intensity), wl.Rmin, wl.dR, wl.dxy), [len(wl.u)])],
+                        len(wl.intensity))
+        res[f'K{K}'] = {'evals_per_s': round(1000.0 / ms, 1), 'ms_per_eval': round(ms, 4),
+                        'alg_bytes_per_eval': nb, 'frac': frac_of(nb, 1000.0 / ms)}
     one = torch.tensor([[1.0, wl.dRA, wl.dDec]], dtype=f64, device=dev)
     c1 = ds.chi2_profile_multi(*args, one, PA=wl.PA, inc=wl.inc, check_central_pixel=False).item()
     c0 = ds.chi2_profile(*args, PA=wl.PA, inc=wl.inc, dRA=wl.dRA, dDec=wl.dDec, check_central_pixel=False).item()
@@ -316,10 +366,13 @@ def extra_configs(ta, g, stream, steps):
     S.hmc(post, x0, n_steps=1, step_size=scale, n_leapfrog=2, generator=gen)       # plan set-up and autotune
     ms = timeit(lambda: S.hmc(post, x0, n_steps=n_steps, step_size=scale, n_leapfrog=n_leap, generator=gen), 2)
     grads = n_steps * n_leap + 1
+    ev = W * grads * 1000.0 / ms
+    nb = eval_bytes([(nxy, swept_rmax(nxy, nR, Rmin, dR, dxy), [len(qs.u)])], nR, grad=True)
     out['hmc_quickstart_posterior_64_chains'] = {
         'workload': "galario's quickstart posterior (Gaussian profile, 6 parameters) on 2e5 visibilities, "
                     f'nxy={nxy}, Nr={nR}, FP64: {W} HMC chains x {n_steps} steps x {n_leap} leapfrog steps',
-        'chain_gradient_evals_per_s': round(W * grads * 1000.0 / ms, 1), 'ms_per_hmc_call': round(ms, 3)}
+        'chain_gradient_evals_per_s': round(ev, 1), 'ms_per_hmc_call': round(ms, 3),
+        'alg_bytes_per_chain_gradient': nb, 'frac': frac_of(nb, ev)}
     plan.close()
     return out
 
@@ -337,6 +390,8 @@ def main():
     ap.add_argument('--batch-grad', type=str, default='1,8,64',
                     help='parameter-set counts of the batched forward+gradient')
     ap.add_argument('--extras', type=int, default=1, help='also time the other configurations (0: skip)')
+    ap.add_argument('--steady-calls', type=int, default=250,
+                    help='calls of the steady-state measurement of each plain form (0: skip)')
     ap.add_argument('--copy-calibration-mb', type=int, default=0,
                     help='first run 5 device-to-device copies of this many MiB (HBM byte-counter calibration)')
     args = ap.parse_args()
@@ -393,19 +448,32 @@ def main():
     stream = torch.cuda.current_stream(dev)
 
     def run_timed(fn, steps, blocks=10):
-        """`steps` calls between events on the kernels' stream, in `blocks` equal blocks:
-        (mean ms per call over all steps, std and min over the blocks' per-call means)."""
-        per = max(1, steps // blocks)
+        """Exactly `steps` calls between events on the kernels' stream, bracketed by synchronisations,
+        in `blocks` blocks of (nearly) equal size: (mean ms per call over all steps, std, min and median
+        over the blocks' per-call means, steps)."""
+        blocks = max(1, min(blocks, steps))
+        sizes = [steps // blocks + (1 if b < steps % blocks else 0) for b in range(blocks)]
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(blocks + 1)]
         torch.cuda.synchronize()
         ev[0].record(stream)
         for b in range(blocks):
-            for _ in range(per):
+            for _ in range(sizes[b]):
                 fn()
             ev[b + 1].record(stream)
         torch.cuda.synchronize()
-        blk = np.array([ev[b].elapsed_time(ev[b + 1]) / per for b in range(blocks)])
-        return float(ev[0].elapsed_time(ev[blocks]) / (per * blocks)), float(blk.std()), float(blk.min()), per * blocks
+        blk = np.array([ev[b].elapsed_time(ev[b + 1]) / sizes[b] for b in range(blocks)])
+        return (float(ev[0].elapsed_time(ev[blocks]) / steps), float(blk.std()), float(blk.min()),
+                float(np.median(blk)), steps)
+
+    def steady_state(fn, blocks=5):
+        """A separate, longer run for a robust per-call figure: --steady-calls calls in `blocks` blocks,
+        the median block's per-call time (and the spread of the blocks)."""
+        if args.steady_calls <= 0:
+            return None
+        mean, std, mn, med, n = run_timed(fn, args.steady_calls, blocks)
+        return {'calls': n, 'blocks': blocks, 'median_block_ms_per_call': round(med, 5),
+                'min_block_ms_per_call': round(mn, 5), 'mean_ms_per_call': round(mean, 5),
+                'evals_per_s_median_block': round(1000.0 / med, 1)}
 
     def stage_breakdown(fn, steps, grad):
         """Per-stage HIP events inside each call (a separate pass: the events add their own gaps)."""
@@ -439,8 +507,9 @@ def main():
         ta.stage_timer(False)
         return stage_ms[stage] / (ncalls * reps)
 
-    ms_per_step, ms_std, ms_min, nsteps = run_timed(step, args.steps)
+    ms_per_step, ms_std, ms_min, _, nsteps = run_timed(step, args.steps)
     evals = 1000.0 / ms_per_step
+    steady = steady_state(step)
     stages, b_alg = stage_breakdown(step, min(args.steps, 50), grad=False)
     kernels = {}
     sb = stage_bytes(wl.nxy, len(wl.u), len(wl.intensity), rmax)
@@ -455,8 +524,9 @@ def main():
 
     for _ in range(args.warmup):
         step_grad()
-    ms_grad, ms_grad_std, ms_grad_min, _ = run_timed(step_grad, args.steps)
+    ms_grad, ms_grad_std, ms_grad_min, _, _ = run_timed(step_grad, args.steps)
     evals_grad = 1000.0 / ms_grad
+    steady_grad = steady_state(step_grad)
     stages_grad, b_alg_grad = stage_breakdown(step_grad, min(args.steps, 50), grad=True)
 
     # the kernel that moves the most HBM bytes per evaluation (the sample kernel: 72 MB against the
@@ -481,9 +551,12 @@ def main():
 
     for _ in range(args.warmup):
         step32()
-    ms32, _, _, _ = run_timed(step32, args.steps)
+    ms32, _, _, _, _ = run_timed(step32, args.steps)
     c32 = float(out.item())
+    b32 = sum(stage_bytes(wl.nxy, len(wl.u), len(wl.intensity), rmax, s=4).values())
     plain_f32 = {'evals_per_s': round(1000.0 / ms32, 2), 'ms_per_step': round(ms32, 5),
+                 'alg_bytes_per_eval': b32, 'frac': frac_of(b32, 1000.0 / ms32),
+                 'steady_state': steady_state(step32),
                  'chi2_rel_err_vs_f64': abs(c32 - chi2_dev) / abs(chi2_dev)}
     del ds32
 
@@ -558,6 +631,7 @@ def main():
                    'nxy': wl.nxy, 'nvis': len(wl.u), 'nr': len(wl.intensity), 'rmax': rmax,
                    'parallelism': 'single GPU'},
         'ms_per_step_std': round(ms_std, 5), 'ms_per_step_min': round(ms_min, 5),
+        'steady_state': steady,
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
                      'alg_bytes_per_launch': kernels[dom]['alg_bytes'],
@@ -573,6 +647,7 @@ def main():
         'stages': stages,
         'forward_grad': {'evals_per_s': round(evals_grad, 2), 'ms_per_step': round(ms_grad, 5),
                          'ms_per_step_std': round(ms_grad_std, 5), 'ms_per_step_min': round(ms_grad_min, 5),
+                         'steady_state': steady_grad,
                          'alg_bytes_per_eval': b_alg_grad,
                          'frac': round(b_alg_grad * evals_grad / (HBM_PEAK_GBS * 1e9), 4),
                          'stages': stages_grad},
