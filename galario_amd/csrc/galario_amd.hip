@@ -519,6 +519,7 @@ struct SampleSetup {
     T cosPA, sinPA;
     int phase;
     T dRA2, dDec2;
+    T dRAt, dDect;   // 2 dRArot, 2 dDecrot: the phase argument in half turns (the double kernels' sincospi)
     T dRArot, dDecrot;
 };
 
@@ -543,6 +544,8 @@ SampleSetup<T> setup_sample(T PA, T dRA, T dDec) {
     s.phase = !(s.dRArot == T(0) && s.dDecrot == T(0));
     s.dRA2 = T(double(s.dRArot) * (2. * double(T(M_PI))));
     s.dDec2 = T(double(s.dDecrot) * (2. * double(T(M_PI))));
+    s.dRAt = T(2) * s.dRArot;
+    s.dDect = T(2) * s.dDecrot;
     return s;
 }
 
@@ -565,6 +568,9 @@ SampleArgs<T> sample_args(const cpx<T>* S, long long rs, long long cs, int nrow,
     a.phase = ss.phase;
     a.dRA2 = ss.dRA2;
     a.dDec2 = ss.dDec2;
+    a.dRAt = ss.dRAt;
+    a.dDect = ss.dDect;
+    a.inv_duv = T(1) / duv;
     a.nd = nd;
     a.u = u;
     a.v = v;
@@ -1307,6 +1313,7 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, int mode, i
                 a.ncol = g.C;
                 a.v_origin = T(1);
                 a.duv = d.duv;
+                a.inv_duv = T(1) / d.duv;
                 a.nd = d.nd;
                 a.u = d.u;
                 a.v = d.v;
@@ -1316,6 +1323,7 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, int mode, i
                 a.flag = flag;
                 a.rho2 = G.ex.rho2;
                 a.duv_d = d.duv_d;
+                a.inv_duv_d = 1.0 / d.duv_d;
                 b.partOff = P.off[si];
                 const dim3 gs(P.nb[si], nw), gm(P.nb[si], (nw + kw - 1) / kw);
                 if constexpr (is_f32) {

@@ -66,18 +66,6 @@ __device__ __forceinline__ void lds_add_f64(double* p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ __forceinline__ int wave_min(int x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x = min(x, __shfl_xor(x, off, 64));
-    return x;
-}
-
-__device__ __forceinline__ int wave_max(int x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x = max(x, __shfl_xor(x, off, 64));
-    return x;
-}
-
 // Storage T (H and the dataset streams of `a`), arithmetic P with the scalars of `sc` (P = T, sc = a:
 // galario's arithmetic in that precision; T = float, P = double: the FP32-bulk form with FP64
 // per-point arithmetic).  The reverse mode runs in double either way.
@@ -97,7 +85,7 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
     for (int j = tid; j < GHOT * GHOT; j += 256) hot[j] = 0.0;
     if (i < a.nd) {
         const P u0 = P(a.u[i]), v0 = P(a.v[i]);
-        const GridPos<P> p = grid_pos(sc, u0, v0);
+        const GridPos<P> p = grid_pos<true>(sc, u0, v0);
         oob = p.oob;
         const P q = p.q, t = p.t;
         const int half = a.nrow / 2;
@@ -115,10 +103,7 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
         amp += (((a0 - a1) + a2) - a3) * t * q;
         const P sg = copysign(P(1), z);
         P cth = P(1), sth = P(0);
-        if (sc.phase) {
-            const P ang = p.urot * sc.dRA2 + p.vrot * sc.dDec2;
-            fsincos(ang, &sth, &cth);
-        }
+        if (sc.phase) phase_factor(sc, p.urot, p.vrot, &sth, &cth);
         const P V0 = copysign(amp, z);
         const P Vre = V0 * cth, Vim = V0 * sth;
         const P w = P(a.w[i]), ore = P(a.re[i]), oim = P(a.im[i]);
@@ -133,7 +118,9 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
         const double gth = Gim * double(Vre) - Gre * double(Vim);
         const double dq = double(1 - t) * double(a3 - a0) + double(t) * double(a2 - a1);
         const double dt = double(1 - q) * double(a1 - a0) + double(q) * double(a2 - a3);
-        const double s_duv = double(p.sign_u) / double(sc.duv);
+        double s_duv;
+        if constexpr (std::is_same<P, double>::value) s_duv = p.sign_u * sc.inv_duv;   // (= sign / duv exactly)
+        else s_duv = double(p.sign_u) / double(sc.duv);
         gpa = gA * (dq * s_duv * (-double(p.vrot)) + dt * double(sc.v_origin) * s_duv * double(p.urot));
         const double two_pi = 6.283185307179586476925286766559;
         gra = gth * two_pi * double(u0);

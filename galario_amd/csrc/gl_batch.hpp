@@ -26,9 +26,10 @@ struct WalkerPrm {
     T cosPA, sinPA;
     int phase;
     T dRA2, dDec2;       // 2 pi dRArot, 2 pi dDecrot
+    T dRAt, dDect;       // 2 dRArot, 2 dDecrot (the phase in half turns)
     double tan_inc;
     // the same rotation and phase scalars in double (the FP32-bulk form's FP64 per-point arithmetic)
-    double cosPAd, sinPAd, dRA2d, dDec2d;
+    double cosPAd, sinPAd, dRA2d, dDec2d, dRAtd, dDectd;
     int phased;
 };
 
@@ -62,6 +63,8 @@ __global__ __launch_bounds__(256) void k_batch_setup(int W, const G* __restrict_
         p.phase = !(dRArot == T(0) && dDecrot == T(0));
         p.dRA2 = T(double(dRArot) * (2. * double(T(M_PI))));
         p.dDec2 = T(double(dDecrot) * (2. * double(T(M_PI))));
+        p.dRAt = T(2) * dRArot;
+        p.dDect = T(2) * dDecrot;
         const double PAd = double(geom[4 * i + 1]), dRAd = double(geom[4 * i + 2]), dDecd = double(geom[4 * i + 3]);
         double dRArotd = dRAd, dDecrotd = dDecd;
         p.cosPAd = 1.0;
@@ -75,6 +78,8 @@ __global__ __launch_bounds__(256) void k_batch_setup(int W, const G* __restrict_
         p.phased = !(dRArotd == 0.0 && dDecrotd == 0.0);
         p.dRA2d = dRArotd * (2. * M_PI);
         p.dDec2d = dDecrotd * (2. * M_PI);
+        p.dRAtd = 2.0 * dRArotd;
+        p.dDectd = 2.0 * dDecrotd;
         wp[i] = p;
     }
     if (gI && i < size_t(W) * size_t(nr)) gI[i] = 0.0;
@@ -133,6 +138,8 @@ __device__ __forceinline__ void walker_sample_args(SampleArgs<T>& a, const Batch
     a.phase = p.phase;
     a.dRA2 = p.dRA2;
     a.dDec2 = p.dDec2;
+    a.dRAt = p.dRAt;
+    a.dDect = p.dDect;
 }
 
 // Scalars of the FP64 per-point arithmetic for walker w on the dataset of `a` (only the fields
@@ -146,6 +153,7 @@ __device__ __forceinline__ SampleArgs<double> walker_scalars_f64(const SampleArg
     sc.ncol = a.ncol;
     sc.v_origin = double(a.v_origin);
     sc.duv = a.duv_d;
+    sc.inv_duv = a.inv_duv_d;
     sc.rho2 = a.rho2;
     sc.rotate = p.rotate;
     sc.cosPA = p.cosPAd;
@@ -153,6 +161,8 @@ __device__ __forceinline__ SampleArgs<double> walker_scalars_f64(const SampleArg
     sc.phase = p.phased;
     sc.dRA2 = p.dRA2d;
     sc.dDec2 = p.dDec2d;
+    sc.dRAt = p.dRAtd;
+    sc.dDect = p.dDectd;
     return sc;
 }
 

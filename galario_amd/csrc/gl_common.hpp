@@ -46,11 +46,57 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
-// Wave64 sum of a double via cross-lane shuffles.
+// Cross-lane moves through DPP row patterns: a modifier of a VALU move, no LDS round trip
+// (shuffles compile to ds_bpermute, which goes through the LDS crossbar).
+enum : int {
+    DPP_XOR1 = 0xB1,             // quad_perm [1,0,3,2]: lane l <- l ^ 1
+    DPP_XOR2 = 0x4E,             // quad_perm [2,3,0,1]: lane l <- l ^ 2
+    DPP_ROW_MIRROR = 0x140,      // lane l <- 15 - l within each row of 16
+    DPP_ROW_HALF_MIRROR = 0x141  // lane l <- 7 - l within each half row of 8
+};
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const int lo = dpp_i32<CTRL>(__double2loint(x)), hi = dpp_i32<CTRL>(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_f64(double x, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                            __builtin_amdgcn_readlane(__double2loint(x), lane));
+}
+
+// Wave64 sum of a double, the same value in every lane; all 64 lanes must be active.  Within each
+// row of 16 lanes by DPP (pairs, quads, half rows, rows: every lane of a row ends with the same
+// row sum), then the four row sums r0..r3 (lanes 0, 16, 32, 48) as (r0 + r1) + (r2 + r3).  Every
+// kernel reduces with this one tree, so sums of the same terms agree bit for bit across kernels.
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
+    x += dpp_f64<DPP_XOR1>(x);
+    x += dpp_f64<DPP_XOR2>(x);
+    x += dpp_f64<DPP_ROW_HALF_MIRROR>(x);
+    x += dpp_f64<DPP_ROW_MIRROR>(x);
+    return (lane_f64(x, 0) + lane_f64(x, 16)) + (lane_f64(x, 32) + lane_f64(x, 48));
+}
+
+// Wave64 minimum / maximum of an int, the same value in every lane (all 64 lanes active).
+__device__ __forceinline__ int wave_min(int x) {
+    x = min(x, dpp_i32<DPP_XOR1>(x));
+    x = min(x, dpp_i32<DPP_XOR2>(x));
+    x = min(x, dpp_i32<DPP_ROW_HALF_MIRROR>(x));
+    x = min(x, dpp_i32<DPP_ROW_MIRROR>(x));
+    return min(min(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16)),
+               min(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
+}
+__device__ __forceinline__ int wave_max(int x) {
+    x = max(x, dpp_i32<DPP_XOR1>(x));
+    x = max(x, dpp_i32<DPP_XOR2>(x));
+    x = max(x, dpp_i32<DPP_ROW_HALF_MIRROR>(x));
+    x = max(x, dpp_i32<DPP_ROW_MIRROR>(x));
+    return max(max(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16)),
+               max(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
 }
 
 }  // namespace gl

@@ -480,6 +480,8 @@ struct SampleArgs {
     T v_origin, duv;
     int rotate;  T cosPA, sinPA;
     int phase;   T dRA2, dDec2;  // 2*pi*dRArot, 2*pi*dDecrot
+    T dRAt, dDect;               // 2*dRArot, 2*dDecrot: the phase in half turns (double: sincospi)
+    T inv_duv;                   // 1/duv (double: the grid position multiplies by it)
     int nd;
     const T* u; const T* v;
     cpx<T>* vis;                 // MODE_VIS
@@ -488,7 +490,7 @@ struct SampleArgs {
     double* partials;            // MODE_CHI2: one per workgroup
     unsigned* flag;              // bit 0: a point fell outside the grid
     double rho2;                 // > 0: squared uv extent in cells; points beyond it count as outside
-    double duv_d;                // duv in double (the FP32-bulk form's FP64 per-point arithmetic)
+    double duv_d, inv_duv_d;     // duv and 1/duv in double (the FP32-bulk form's FP64 per-point arithmetic)
 };
 
 // Grid coordinates of a (rotated) point; oob when it is not inside the grid
@@ -500,7 +502,11 @@ struct GridPos {
     bool oob;
 };
 
-template <typename T>
+// RECIP (the real, even spectrum, in double): the two divisions by duv become multiplications by
+// 1/duv, within an ulp of the quotients (4e-13 cells at the grid's edge).  The interpolated
+// phase is then 0 or pi, so the sample moves by that ulp only.  The general complex path keeps
+// the divisions: galario's phase atan2(Z) amplifies a change of the position by 1/|Z|.
+template <bool RECIP = false, typename T>
 __device__ __forceinline__ GridPos<T> grid_pos(const SampleArgs<T>& a, T u0, T v0) {
     GridPos<T> g;
     g.urot = u0;
@@ -509,9 +515,15 @@ __device__ __forceinline__ GridPos<T> grid_pos(const SampleArgs<T>& a, T u0, T v
         g.urot = u0 * a.cosPA - v0 * a.sinPA;
         g.vrot = u0 * a.sinPA + v0 * a.cosPA;
     }
-    const T indu = fabs(g.urot) / a.duv;
     g.sign_u = copysign(T(1), g.urot);
-    const T indv = T(a.nrow / 2) + a.v_origin * g.sign_u * g.vrot / a.duv;
+    T indu, indv;
+    if constexpr (RECIP && std::is_same<T, double>::value) {
+        indu = fabs(g.urot) * a.inv_duv;
+        indv = T(a.nrow / 2) + a.v_origin * g.sign_u * g.vrot * a.inv_duv;
+    } else {
+        indu = fabs(g.urot) / a.duv;
+        indv = T(a.nrow / 2) + a.v_origin * g.sign_u * g.vrot / a.duv;
+    }
     g.oob = !(indu <= T(a.ncol - 1) && indv >= T(0) && indv <= T(a.nrow - 1));
     if (a.rho2 > 0.0) {   // beyond the declared extent: the spectrum there may not have been stored
         const double du = double(indu), dv = double(indv) - double(a.nrow / 2);
@@ -526,12 +538,23 @@ __device__ __forceinline__ GridPos<T> grid_pos(const SampleArgs<T>& a, T u0, T v
     return g;
 }
 
+// e^{i theta}, theta = urot 2 pi dRArot + vrot 2 pi dDecrot.  In double as sincospi of theta / pi
+// (argument reduction by exact subtraction instead of the division by 2 pi of sincos); the angle
+// agrees to an ulp.
+template <typename T>
+__device__ __forceinline__ void phase_factor(const SampleArgs<T>& a, T urot, T vrot, T* s, T* c) {
+    if constexpr (std::is_same<T, double>::value) {
+        sincospi(urot * a.dRAt + vrot * a.dDect, s, c);
+    } else {
+        fsincos(urot * a.dRA2 + vrot * a.dDec2, s, c);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ cpx<T> apply_phase_pt(const SampleArgs<T>& a, const GridPos<T>& g, cpx<T> V) {
     if (a.phase) {
-        const T ang = g.urot * a.dRA2 + g.vrot * a.dDec2;
         T s, c;
-        fsincos(ang, &s, &c);
+        phase_factor(a, g.urot, g.vrot, &s, &c);
         V = cpx<T>{V.re * c - V.im * s, V.re * s + V.im * c};
     }
     return V;
@@ -578,7 +601,7 @@ __device__ __forceinline__ cpx<T> sample_point(const SampleArgs<T>& a, T u0, T v
 template <typename T, typename P>
 __device__ __forceinline__ cpx<P> sample_point_sym2(const SampleArgs<T>& a, const SampleArgs<P>& sc, P u0, P v0,
                                                     bool& oob) {
-    const GridPos<P> g = grid_pos(sc, u0, v0);
+    const GridPos<P> g = grid_pos<true>(sc, u0, v0);
     oob = g.oob;
     const P q = g.q, t = g.t;
     const int half = a.nrow / 2;
@@ -728,10 +751,13 @@ __device__ __forceinline__ cpx<T> multisrc_point(const SampleArgs<T>& a, int i, 
         }
         cpx<T> Vk = V0;
         if (!(dRAr == T(0) && dDecr == T(0))) {
-            const T ang = ur * T(double(dRAr) * (2. * double(T(M_PI)))) +
-                          vr * T(double(dDecr) * (2. * double(T(M_PI))));
+            SampleArgs<T> ak;          // this source's phase scalars, as setup_sample forms them
+            ak.dRA2 = T(double(dRAr) * (2. * double(T(M_PI))));
+            ak.dDec2 = T(double(dDecr) * (2. * double(T(M_PI))));
+            ak.dRAt = T(2) * dRAr;
+            ak.dDect = T(2) * dDecr;
             T sn, cs;
-            fsincos(ang, &sn, &cs);
+            phase_factor(ak, ur, vr, &sn, &cs);
             Vk = cpx<T>{V0.re * cs - V0.im * sn, V0.re * sn + V0.im * cs};
         }
         V.re += f * Vk.re;
