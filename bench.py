@@ -175,7 +175,8 @@ def batched_rates(ta, ds, wl, I, chi2_plain, counts, counts_grad, steps, stream)
         ev = P * 1000.0 / ms
         nb = eval_bytes(grids, nr, kw=kw)
         out['forward'][str(P)] = {'evals_per_s': round(ev, 1), 'ms_per_batch': round(ms, 4),
-                                  'chunk': chunk, 'walkers_per_sample': kw, 'chi2_rel_diff_vs_plain': rel,
+                                  'chunk': chunk, 'walkers_per_sample': kw, 'point_blocks': plan.tile(P)[0],
+                                  'chi2_rel_diff_vs_plain': rel,
                                   'alg_bytes_per_eval': nb, 'frac': frac_of(nb, ev)}
     for P in counts_grad:
         Ip, Gp = It[:P].contiguous(), Gt[:P].contiguous()
@@ -185,7 +186,8 @@ def batched_rates(ta, ds, wl, I, chi2_plain, counts, counts_grad, steps, stream)
         ev = P * 1000.0 / ms
         nb = eval_bytes(grids, nr, grad=True)
         out['forward_grad'][str(P)] = {'evals_per_s': round(ev, 1), 'ms_per_batch': round(ms, 4),
-                                       'chunk': chunk, 'alg_bytes_per_eval': nb, 'frac': frac_of(nb, ev)}
+                                       'chunk': chunk, 'grad_window': plan.tile(P, grad=True)[1],
+                                       'alg_bytes_per_eval': nb, 'frac': frac_of(nb, ev)}
     plan.close()
     return out
 
@@ -268,7 +270,8 @@ def extra_configs(ta, g, stream, steps):
         'workload': '256 walkers (Nr=300) x datasets (nxy, Nvis) = (512, 1e4), (1024, 2e5), (2048, 1e6), FP64, '
                     'per-walker chi2 summed over the datasets',
         'walker_evals_per_s': round(ev, 1), 'ms_per_batch': round(ms, 4), 'chunk': chunk,
-        'walkers_per_sample': kw, 'alg_bytes_per_walker_eval': nb, 'frac': frac_of(nb, ev),
+        'walkers_per_sample': kw, 'point_blocks': plan.tile(256)[0], 'alg_bytes_per_walker_eval': nb,
+        'frac': frac_of(nb, ev),
         'chi2_rel_diff_vs_plain': float(np.max(np.abs(per - ref) / np.abs(ref)))}
     plan.close()
     del sets
@@ -309,6 +312,7 @@ def extra_configs(ta, g, stream, steps):
                  max(e[1], float(torch.linalg.norm(gI[w] - gI_ref) / torch.linalg.norm(gI_ref))),
                  max(e[2], float(torch.max(torch.abs(gg[w] - gg_ref) / torch.abs(gg_ref))))]
         res[name] = {'fwd_grad_set_evals_per_s': round(ev, 1), 'ms_per_batch': round(ms, 4), 'chunk': chunk,
+                     'grad_window': plan.tile(64, grad=True)[1],
                      'alg_bytes_per_set_eval': nb, 'frac': frac_of(nb, ev),
                      'chi2_rel_err_vs_f64_plain': e[0], 'grad_I_rel_err_vs_f64_plain': e[1],
                      'grad_geom_max_rel_err_vs_f64_plain': e[2]}

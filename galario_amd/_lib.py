@@ -81,6 +81,8 @@ def _load():
     lib.galario_batch_destroy.argtypes = [_vp]
     lib.galario_batch_info.argtypes = [_vp, _c_int, _c_int, _ip, _ip, _ip]
     lib.galario_batch_set_chunk.argtypes = [_vp, _c_int, _c_int, _c_int, _c_int]
+    lib.galario_batch_tile.argtypes = [_vp, _c_int, _c_int, _ip, _ip]
+    lib.galario_batch_set_tile.argtypes = [_vp, _c_int, _c_int, _c_int, _c_int]
     lib.galario_batch_take_oob_flag.argtypes = [_vp, _ip]
     lib.galario_batch_chi2_profile_f32x.argtypes = [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _c_int]
     lib.galario_batch_chi2_profile_grad_f32x.argtypes = [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int]

@@ -1170,6 +1170,9 @@ struct BatchPlan : BatchPlanBase {
     struct Exec {
         int chunk = 0;
         int kw = 4;                    // walkers per sample workgroup (forward)
+        int ppt = 1;                   // 256-point blocks per sample workgroup (forward)
+        int win = GWIN;                // LDS window cells of the gradient sample kernel
+        bool tile_fixed = false;       // ppt / win set by the caller: the autotuner keeps them
         hipGraphExec_t exec = nullptr;
     };
     std::map<std::pair<int, int>, Exec> runs;   // (nwalkers, mode)
@@ -1257,20 +1260,47 @@ BatchPtrs<T> batch_ptrs(BatchPlan<T>& P, bool grad) {
 
 // Enqueue one batched evaluation of W walkers (parameters already in P.I /
 // P.geom) in chunks of `chunk` walkers; only kernel launches (capturable).
+template <typename T, bool MIX, int KW>
+void launch_sample_multi_kw(int ppt, int nb, int nw, hipStream_t s, const SampleArgs<T>& a, const BatchPtrs<T>& b) {
+    const dim3 grid((nb + ppt - 1) / ppt, (nw + KW - 1) / KW);
+    switch (ppt) {
+        case 1: hipLaunchKernelGGL((k_sample_multi_b<T, KW, MIX, 1>), grid, dim3(256), 0, s, a, b, nw, nb); break;
+        case 2: hipLaunchKernelGGL((k_sample_multi_b<T, KW, MIX, 2>), grid, dim3(256), 0, s, a, b, nw, nb); break;
+        default: fail(GALARIO_ERR_RUNTIME, "internal: point blocks per sample workgroup must be 1 or 2");
+    }
+}
+
 template <typename T, bool MIX>
-void launch_sample_multi(int kw, dim3 grid, hipStream_t s, const SampleArgs<T>& a, const BatchPtrs<T>& b, int nw) {
+void launch_sample_multi(int kw, int ppt, int nb, int nw, hipStream_t s, const SampleArgs<T>& a,
+                         const BatchPtrs<T>& b) {
     switch (kw) {
-        case 1: hipLaunchKernelGGL((k_sample_multi_b<T, 1, MIX>), grid, dim3(256), 0, s, a, b, nw); break;
-        case 2: hipLaunchKernelGGL((k_sample_multi_b<T, 2, MIX>), grid, dim3(256), 0, s, a, b, nw); break;
-        case 4: hipLaunchKernelGGL((k_sample_multi_b<T, 4, MIX>), grid, dim3(256), 0, s, a, b, nw); break;
-        case 8: hipLaunchKernelGGL((k_sample_multi_b<T, 8, MIX>), grid, dim3(256), 0, s, a, b, nw); break;
+        case 1: launch_sample_multi_kw<T, MIX, 1>(ppt, nb, nw, s, a, b); break;
+        case 2: launch_sample_multi_kw<T, MIX, 2>(ppt, nb, nw, s, a, b); break;
+        case 4: launch_sample_multi_kw<T, MIX, 4>(ppt, nb, nw, s, a, b); break;
+        case 8: launch_sample_multi_kw<T, MIX, 8>(ppt, nb, nw, s, a, b); break;
         default: fail(GALARIO_ERR_RUNTIME, "internal: walkers per sample workgroup must be 1, 2, 4 or 8");
     }
 }
 
+template <typename T, bool MIX>
+void launch_sample_grad(int win, int nb, int nw, hipStream_t s, const SampleArgs<T>& a, const BatchPtrs<T>& b) {
+    const dim3 grid(nb, nw);
+    switch (win) {
+        case 2048: hipLaunchKernelGGL((k_sample_grad_sym_b<T, MIX, 2048>), grid, dim3(256), 0, s, a, b); break;
+        case GWIN: hipLaunchKernelGGL((k_sample_grad_sym_b<T, MIX, GWIN>), grid, dim3(256), 0, s, a, b); break;
+        case 4096: hipLaunchKernelGGL((k_sample_grad_sym_b<T, MIX, 4096>), grid, dim3(256), 0, s, a, b); break;
+        default: fail(GALARIO_ERR_RUNTIME, "internal: gradient window must be 2048, 3072 or 4096 cells");
+    }
+}
+
+// Launch shape of one batched evaluation.
+struct BatchShape {
+    int chunk, kw, ppt, win;
+};
+
 template <typename T>
-void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, int mode, int kw) {
-    const int nr = P.nr;
+void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, const BatchShape& sh, int mode) {
+    const int nr = P.nr, chunk = sh.chunk;
     constexpr bool is_f32 = std::is_same<T, float>::value;   // (the mixed form exists for float plans only)
     const bool grad = mode & MODE_GRAD, mix = is_f32 && (mode & MODE_F64_PHASE);
     const size_t nz = std::max(size_t(W), grad ? size_t(W) * size_t(std::max(nr, 4)) : size_t(W));
@@ -1325,13 +1355,13 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, int chunk, int mode, i
                 a.duv_d = d.duv_d;
                 a.inv_duv_d = 1.0 / d.duv_d;
                 b.partOff = P.off[si];
-                const dim3 gs(P.nb[si], nw), gm(P.nb[si], (nw + kw - 1) / kw);
+                const int nb = P.nb[si];
                 if constexpr (is_f32) {
-                    if (grad && mix) hipLaunchKernelGGL((k_sample_grad_sym_b<T, true>), gs, dim3(256), 0, s, a, b);
-                    else if (mix) launch_sample_multi<T, true>(kw, gm, s, a, b, nw);
+                    if (grad && mix) launch_sample_grad<T, true>(sh.win, nb, nw, s, a, b);
+                    else if (mix) launch_sample_multi<T, true>(sh.kw, sh.ppt, nb, nw, s, a, b);
                 }
-                if (grad && !mix) hipLaunchKernelGGL((k_sample_grad_sym_b<T, false>), gs, dim3(256), 0, s, a, b);
-                else if (!mix) launch_sample_multi<T, false>(kw, gm, s, a, b, nw);
+                if (grad && !mix) launch_sample_grad<T, false>(sh.win, nb, nw, s, a, b);
+                else if (!mix) launch_sample_multi<T, false>(sh.kw, sh.ppt, nb, nw, s, a, b);
             }
             if (grad) {
                 AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0, nullptr, 0, Dst4{}, ablocks,
@@ -1460,17 +1490,17 @@ BatchPlan<T>& plan_of(void* h) {
     return P;
 }
 
-// Average time of one eager evaluation at `chunk` (after one warm-up).
+// Average time of one eager evaluation of shape `sh` (after one warm-up).
 template <typename T>
-float batch_time(BatchPlan<T>& P, hipStream_t s, int W, int chunk, int mode, int kw) {
-    P.ensure_slots(chunk, mode & MODE_GRAD, s);
-    batch_enqueue(P, s, W, chunk, mode, kw);
+float batch_time(BatchPlan<T>& P, hipStream_t s, int W, const BatchShape& sh, int mode) {
+    P.ensure_slots(sh.chunk, mode & MODE_GRAD, s);
+    batch_enqueue(P, s, W, sh, mode);
     hipEvent_t e0, e1;
     HC(hipEventCreate(&e0));
     HC(hipEventCreate(&e1));
     HC(hipEventRecord(e0, s));
     const int reps = 2;
-    for (int r = 0; r < reps; ++r) batch_enqueue(P, s, W, chunk, mode, kw);
+    for (int r = 0; r < reps; ++r) batch_enqueue(P, s, W, sh, mode);
     HC(hipEventRecord(e1, s));
     HC(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1481,35 +1511,64 @@ float batch_time(BatchPlan<T>& P, hipStream_t s, int W, int chunk, int mode, int
 }
 
 // Launch shape for W walkers, by timing eager evaluations on the caller's
-// stream: the chunk size (powers of two and W itself, within the slot
-// budget) at 4 walkers per sample workgroup, then (forward) the walkers per
-// sample workgroup (1, 2, 4, 8) at that chunk size.
+// stream, one dimension at a time from (4 walkers per sample workgroup, one
+// 256-point block per workgroup, 3072-cell gradient window): the chunk size
+// (powers of two and W itself, within the slot budget); forward: the walkers
+// per sample workgroup (1, 2, 4, 8), then the point blocks per workgroup
+// (1, 2); gradient: the LDS window (2048, 3072, 4096 cells).  Point blocks and
+// window set by the caller (tile_fixed) are kept.
 template <typename T>
-void batch_tune(BatchPlan<T>& P, hipStream_t s, int W, int mode, int* chunk, int* kw) {
+void batch_tune(BatchPlan<T>& P, hipStream_t s, int W, int mode, typename BatchPlan<T>::Exec& run) {
     const bool grad = mode & MODE_GRAD;
     const int cmax = P.max_chunk(grad, W);
     std::vector<int> cand;
     for (int c = 1; c < cmax; c *= 2) cand.push_back(c);
     cand.push_back(cmax);
-    int best = cand.back(), bkw = 4;
+    BatchShape best{cand.back(), 4, run.tile_fixed ? run.ppt : 1, run.tile_fixed ? run.win : GWIN};
     float tbest = 0.f;
     for (int c : cand) {
-        const float t = batch_time(P, s, W, c, mode, 4);
-        if (c == cand.front() || t < tbest) {
-            tbest = t;
-            best = c;
+        BatchShape t = best;
+        t.chunk = c;
+        const float ms = batch_time(P, s, W, t, mode);
+        if (c == cand.front() || ms < tbest) {
+            tbest = ms;
+            best.chunk = c;
         }
     }
-    if (!grad)
-        for (int k : {1, 2, 8}) {
-            const float t = batch_time(P, s, W, best, mode, k);
-            if (t < tbest) {
-                tbest = t;
-                bkw = k;
-            }
+    auto try_shape = [&](BatchShape t) {
+        const float ms = batch_time(P, s, W, t, mode);
+        if (ms < tbest) {
+            tbest = ms;
+            best = t;
         }
-    *chunk = best;
-    *kw = bkw;
+    };
+    if (!grad) {
+        for (int k : {1, 2, 8}) {
+            BatchShape t = best;
+            t.kw = k;
+            if (k != best.kw) try_shape(t);
+        }
+        if (!run.tile_fixed) {
+            BatchShape t = best;
+            t.ppt = 2;
+            try_shape(t);
+        }
+    } else if (!run.tile_fixed) {
+        for (int wn : {2048, 4096}) {
+            BatchShape t = best;
+            t.win = wn;
+            try_shape(t);
+        }
+    }
+    run.chunk = best.chunk;
+    run.kw = best.kw;
+    run.ppt = best.ppt;
+    run.win = best.win;
+}
+
+template <typename T>
+BatchShape shape_of(const typename BatchPlan<T>::Exec& run) {
+    return BatchShape{run.chunk, run.kw, run.ppt, run.win};
 }
 
 // geom of type G: T, or double with MODE_F64_PHASE (float plans only).
@@ -1543,10 +1602,11 @@ int batch_run(void* h, void* stream, int W, const T* I, const G* geom, double* c
             HC(hipMemcpyAsync(P.geom.p, geom, sizeof(G) * size_t(W) * 4, hipMemcpyDeviceToDevice, s));
         const auto key = std::make_pair(W, mode);
         auto& run = P.runs[key];
-        if (run.chunk == 0) batch_tune(P, s, W, mode, &run.chunk, &run.kw);
+        if (run.chunk == 0) batch_tune(P, s, W, mode, run);
         P.ensure_slots(run.chunk, grad, s);
+        const BatchShape sh = shape_of<T>(run);
         if (flags & GALARIO_BATCH_EAGER) {
-            batch_enqueue(P, s, W, run.chunk, mode, run.kw);
+            batch_enqueue(P, s, W, sh, mode);
         } else {
             if (!run.exec) {
                 // capture on the plan's stream (nothing runs during capture)
@@ -1554,7 +1614,7 @@ int batch_run(void* h, void* stream, int W, const T* I, const G* geom, double* c
                 hipGraph_t graph = nullptr;
                 HC(hipStreamBeginCapture(P.cap, hipStreamCaptureModeThreadLocal));
                 try {
-                    batch_enqueue(P, P.cap, W, run.chunk, mode, run.kw);
+                    batch_enqueue(P, P.cap, W, sh, mode);
                 } catch (...) {
                     (void)hipStreamEndCapture(P.cap, &graph);
                     if (graph) (void)hipGraphDestroy(graph);
@@ -1930,6 +1990,51 @@ int galario_batch_set_chunk(void* plan, int nwalkers, int mode, int chunk, int w
             run.exec = nullptr;
             run.chunk = chunk;
             run.kw = walkers_per_sample;
+        };
+        if (b->prec == 8) set(*static_cast<BatchPlan<double>*>(b));
+        else set(*static_cast<BatchPlan<float>*>(b));
+    });
+}
+
+int galario_batch_tile(void* plan, int nwalkers, int mode, int* point_blocks, int* grad_window) {
+    return guard([&] {
+        auto* b = static_cast<BatchPlanBase*>(plan);
+        if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
+        int pb = 0, wn = 0;
+        auto look = [&](auto& P) {
+            std::lock_guard<std::mutex> lk(P.mu);
+            auto it = P.runs.find(std::make_pair(nwalkers, mode));
+            if (it != P.runs.end() && (it->second.chunk > 0 || it->second.tile_fixed)) {
+                pb = (mode & MODE_GRAD) ? 1 : it->second.ppt;
+                wn = (mode & MODE_GRAD) ? it->second.win : 0;
+            }
+        };
+        if (b->prec == 8) look(*static_cast<BatchPlan<double>*>(b));
+        else look(*static_cast<BatchPlan<float>*>(b));
+        if (point_blocks) *point_blocks = pb;
+        if (grad_window) *grad_window = wn;
+    });
+}
+
+int galario_batch_set_tile(void* plan, int nwalkers, int mode, int point_blocks, int grad_window) {
+    return guard([&] {
+        auto* b = static_cast<BatchPlanBase*>(plan);
+        if (!b || b->magic != BATCH_MAGIC) invalid("batch", "not a batch plan");
+        auto set = [&](auto& P) {
+            std::lock_guard<std::mutex> lk(P.mu);
+            if (nwalkers < 1 || nwalkers > P.max_w) invalid("batch", "nwalkers out of range");
+            if (mode < 0 || mode > (MODE_GRAD | MODE_F64_PHASE) || ((mode & MODE_F64_PHASE) && P.prec != 4))
+                invalid("batch", "mode = " + std::to_string(mode) + " (gradient 1, FP64 phase 2 on float plans)");
+            if (point_blocks != 1 && point_blocks != 2)
+                invalid("batch", "point blocks per sample workgroup must be 1 or 2");
+            if (grad_window != 2048 && grad_window != GWIN && grad_window != 4096)
+                invalid("batch", "gradient window must be 2048, 3072 or 4096 cells");
+            auto& run = P.runs[std::make_pair(nwalkers, mode)];
+            if (run.exec) (void)hipGraphExecDestroy(run.exec);
+            run.exec = nullptr;
+            run.ppt = point_blocks;
+            run.win = grad_window;
+            run.tile_fixed = true;
         };
         if (b->prec == 8) set(*static_cast<BatchPlan<double>*>(b));
         else set(*static_cast<BatchPlan<float>*>(b));

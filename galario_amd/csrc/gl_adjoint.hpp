@@ -52,6 +52,7 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtom
 constexpr int GHOT = 16;          // hot tile: columns [0, GHOT) x folded rows [0, GHOT)
 constexpr int GREP = 32;          // replicas of the hot tile in global memory
 constexpr int GWIN = 3072;        // LDS window capacity (cells): 26 KB with the hot tile, 6 waves per SIMD
+                                  // (the single call's; the batched form's autotuner also tries 2048 and 4096)
 constexpr int GHOT_ELEMS = GREP * GHOT * GHOT;
 
 template <typename T>
@@ -68,11 +69,13 @@ __device__ __forceinline__ void lds_add_f64(double* p, double v) {
 
 // Storage T (H and the dataset streams of `a`), arithmetic P with the scalars of `sc` (P = T, sc = a:
 // galario's arithmetic in that precision; T = float, P = double: the FP32-bulk form with FP64
-// per-point arithmetic).  The reverse mode runs in double either way.
-template <typename T, typename P>
+// per-point arithmetic).  The reverse mode runs in double either way.  WIN: LDS window capacity
+// (cells); a larger window keeps more workgroups off the direct atomics, a smaller one leaves room
+// for more workgroups per CU.
+template <typename T, typename P, int WIN = GWIN>
 __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, const SampleArgs<P>& sc,
                                                      const GradArgs<T>& g, int blk) {
-    __shared__ double win[GWIN];
+    __shared__ double win[WIN];
     __shared__ double hot[GHOT * GHOT];
     __shared__ double red[4][4];
     __shared__ int box[4][4];
@@ -156,7 +159,7 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
     const int mmin = min(min(box[0][2], box[1][2]), min(box[2][2], box[3][2]));
     const int mmax = max(max(box[0][3], box[1][3]), max(box[2][3], box[3][3]));
     const int wp = pmax - pmin + 1, wm = mmax - mmin + 1, w2 = 2 * wm;
-    const bool use_win = pmax >= 0 && wp * w2 <= GWIN;     // (wp, wm <= 2^13, no overflow)
+    const bool use_win = pmax >= 0 && wp * w2 <= WIN;      // (wp, wm <= 2^13, no overflow)
     const int nwin = use_win ? wp * w2 : 0;
     for (int j = tid; j < nwin; j += 256) win[j] = 0.0;
     __syncthreads();

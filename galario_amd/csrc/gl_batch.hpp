@@ -167,66 +167,77 @@ __device__ __forceinline__ SampleArgs<double> walker_scalars_f64(const SampleArg
 }
 
 // KW walkers per workgroup on the same points: the dataset streams (u, v, Re,
-// Im, w) are read once for KW walkers.  Each walker's per-workgroup partial
-// is the same fixed-order sum as in the one-walker kernel.
+// Im, w) are read once for KW walkers.  PPT blocks of 256 points per workgroup
+// (more loads in flight per wave); each block's partial per walker is the same
+// fixed-order sum as in the one-walker kernel, whatever KW and PPT.
 //
 // MIX (T = float only): FP32 storage with the position, interpolation, phase and residual in FP64
 // (walker_scalars_f64).
-template <typename T, int KW, bool MIX = false>
-__global__ __launch_bounds__(256) void k_sample_multi_b(SampleArgs<T> a, BatchPtrs<T> b, int nw) {
+template <typename T, int KW, bool MIX = false, int PPT = 1>
+__global__ __launch_bounds__(256) void k_sample_multi_b(SampleArgs<T> a, BatchPtrs<T> b, int nw, int nblocks) {
     using P = std::conditional_t<MIX, double, T>;
-    __shared__ double red[KW][4];
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    __shared__ double red[PPT][KW][4];
     const int s0 = blockIdx.y * KW;
-    double acc[KW];
+    double acc[PPT][KW];
     bool oob_any = false;
 #pragma unroll
-    for (int k = 0; k < KW; ++k) acc[k] = 0.0;
-    if (i < a.nd) {
-        const P u0 = P(a.u[i]), v0 = P(a.v[i]);
-        const P re = P(a.re[i]), im = P(a.im[i]);
-        const P sw = sqrt(P(a.w[i]));
+    for (int q = 0; q < PPT; ++q) {
 #pragma unroll
-        for (int k = 0; k < KW; ++k) {
-            const int slot = s0 + k;
-            if (slot < nw) {
-                SampleArgs<T> ak = a;
-                walker_sample_args(ak, b, slot, b.w0 + slot);
-                bool oob = false;
-                cpx<P> V;
-                if constexpr (MIX) V = sample_point_sym2<T, double>(ak, walker_scalars_f64(a, b, b.w0 + slot), u0, v0, oob);
-                else V = sample_point_sym(ak, u0, v0, oob);
-                oob_any |= oob;
-                const P cr = (V.re - re) * sw;
-                const P ci = (V.im - im) * sw;
-                if constexpr (std::is_same<P, double>::value) acc[k] = cr * cr + ci * ci;
-                else acc[k] = double(cr) * cr + double(ci) * ci;
+        for (int k = 0; k < KW; ++k) acc[q][k] = 0.0;
+        const int i = (blockIdx.x * PPT + q) * 256 + threadIdx.x;
+        if (i < a.nd) {
+            const P u0 = P(a.u[i]), v0 = P(a.v[i]);
+            const P re = P(a.re[i]), im = P(a.im[i]);
+            const P sw = sqrt(P(a.w[i]));
+#pragma unroll
+            for (int k = 0; k < KW; ++k) {
+                const int slot = s0 + k;
+                if (slot < nw) {
+                    SampleArgs<T> ak = a;
+                    walker_sample_args(ak, b, slot, b.w0 + slot);
+                    bool oob = false;
+                    cpx<P> V;
+                    if constexpr (MIX)
+                        V = sample_point_sym2<T, double>(ak, walker_scalars_f64(a, b, b.w0 + slot), u0, v0, oob);
+                    else
+                        V = sample_point_sym(ak, u0, v0, oob);
+                    oob_any |= oob;
+                    const P cr = (V.re - re) * sw;
+                    const P ci = (V.im - im) * sw;
+                    if constexpr (std::is_same<P, double>::value) acc[q][k] = cr * cr + ci * ci;
+                    else acc[q][k] = double(cr) * cr + double(ci) * ci;
+                }
             }
         }
     }
     if (oob_any && a.flag) atomicOr(a.flag, 1u);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < KW; ++k) {
-        const double x = wave_sum(acc[k]);
-        if (lane == 0) red[k][wv] = x;
+    for (int q = 0; q < PPT; ++q) {
+#pragma unroll
+        for (int k = 0; k < KW; ++k) {
+            const double x = wave_sum(acc[q][k]);
+            if (lane == 0) red[q][k][wv] = x;
+        }
     }
     __syncthreads();
-    if (threadIdx.x < KW && s0 + int(threadIdx.x) < nw) {
-        const int k = threadIdx.x;
-        b.part[size_t(b.w0 + s0 + k) * 4 * b.ldPart + b.partOff + blockIdx.x] =
-            (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+    if (threadIdx.x < PPT * KW) {
+        const int q = threadIdx.x / KW, k = threadIdx.x % KW;
+        const int blk = blockIdx.x * PPT + q;
+        if (s0 + k < nw && blk < nblocks)
+            b.part[size_t(b.w0 + s0 + k) * 4 * b.ldPart + b.partOff + blk] =
+                (red[q][k][0] + red[q][k][1]) + (red[q][k][2] + red[q][k][3]);
     }
 }
 
-template <typename T, bool MIX = false>
+template <typename T, bool MIX = false, int WIN = GWIN>
 __global__ __launch_bounds__(256) void k_sample_grad_sym_b(SampleArgs<T> a, BatchPtrs<T> b) {
     const int slot = blockIdx.y, w = b.w0 + slot;
     walker_sample_args(a, b, slot, w);
     double* hh = b.Hh + slot * b.sHh;
     const GradArgs<T> g{hh, hh + b.hot_off, b.part + size_t(w) * 4 * b.ldPart + b.partOff, b.ldPart};
-    if constexpr (MIX) sample_grad_sym_body<T, double>(a, walker_scalars_f64(a, b, w), g, blockIdx.x);
-    else sample_grad_sym_body<T, T>(a, a, g, blockIdx.x);
+    if constexpr (MIX) sample_grad_sym_body<T, double, WIN>(a, walker_scalars_f64(a, b, w), g, blockIdx.x);
+    else sample_grad_sym_body<T, T, WIN>(a, a, g, blockIdx.x);
 }
 
 template <typename T, int LOG2N>

@@ -441,6 +441,18 @@ class BatchedChi2Profile:
         _check(lib.galario_batch_set_chunk(self._h, int(nwalkers), self._mode(grad), int(chunk),
                                            int(walkers_per_sample)))
 
+    def tile(self, nwalkers, grad=False):
+        """(256-point blocks per forward sample workgroup, LDS window cells of the gradient sample kernel)
+        picked for nwalkers; zeros before the first call."""
+        pb, wn = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.galario_batch_tile(self._h, int(nwalkers), self._mode(grad), ctypes.byref(pb), ctypes.byref(wn)))
+        return pb.value, wn.value
+
+    def set_tile(self, nwalkers, point_blocks=1, grad_window=3072, grad=False):
+        """Fix the sample kernel's tile for nwalkers (the first call's timing keeps it)."""
+        _check(lib.galario_batch_set_tile(self._h, int(nwalkers), self._mode(grad), int(point_blocks),
+                                          int(grad_window)))
+
     def take_oob_flag(self) -> bool:
         out = ctypes.c_int(0)
         with torch.cuda.device(self.device):

@@ -352,7 +352,7 @@ int galario_batch_chi2_profile_grad_f32x(void *plan, void *stream, int nwalkers,
                                          const double *geom, double *chi2_sets, double *chi2, double *grad_I,
                                          double *grad_geom, int flags);
 
-/* Launch modes of a plan (the `mode` of galario_batch_info / _set_chunk):
+/* Launch modes of a plan (the `mode` of galario_batch_info, _set_chunk, _tile, _set_tile):
  * each mode has its own timed launch shape and captured graph. */
 #define GALARIO_BATCH_MODE_GRAD 1       /* the _grad calls */
 #define GALARIO_BATCH_MODE_F64_PHASE 2  /* the _f32x calls (float plans) */
@@ -365,6 +365,19 @@ int galario_batch_info(void *plan, int nwalkers, int mode, int *chunk, int *walk
 /* Overrides the timed choice for (nwalkers, mode): chunk size and walkers
  * per sample workgroup (1, 2, 4 or 8). */
 int galario_batch_set_chunk(void *plan, int nwalkers, int mode, int chunk, int walkers_per_sample);
+/* The tile of (nwalkers, mode)'s sample kernel, chosen with the chunk
+ * (0 before the first call): forward -- point_blocks, the 256-point blocks
+ * per sample workgroup (1 or 2; each block's partial is the same fixed-order
+ * sum either way, so the outputs do not depend on it), grad_window 0;
+ * gradient -- point_blocks 1 and grad_window, the LDS window of the
+ * corner-gradient scatter in cells (2048, 3072 or 4096; a workgroup whose
+ * corners fit it accumulates there, others add directly to the half
+ * spectrum).  Any pointer may be null. */
+int galario_batch_tile(void *plan, int nwalkers, int mode, int *point_blocks, int *grad_window);
+/* Fixes the tile of (nwalkers, mode) (point_blocks 1 or 2, grad_window 2048,
+ * 3072 or 4096): the timed choice of the first call keeps it and tunes the
+ * rest of the launch shape. */
+int galario_batch_set_tile(void *plan, int nwalkers, int mode, int point_blocks, int grad_window);
 /* Synchronises the plan's last stream; returns (and clears) its out-of-grid flag. */
 int galario_batch_take_oob_flag(void *plan, int *out);
 

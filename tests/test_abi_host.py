@@ -160,6 +160,10 @@ def test_batch_plan_argument_checks_before_any_gpu_work():
         _lib.check(status([ds(512, 0.01 * a)], max_w=0))
     with pytest.raises(ValueError, match="not a batch plan"):
         _lib.check(_lib.lib.galario_batch_info(ctypes.c_void_p(0), 1, 0, None, None, None))
+    with pytest.raises(ValueError, match="not a batch plan"):
+        _lib.check(_lib.lib.galario_batch_tile(ctypes.c_void_p(0), 1, 0, None, None))
+    with pytest.raises(ValueError, match="not a batch plan"):
+        _lib.check(_lib.lib.galario_batch_set_tile(ctypes.c_void_p(0), 1, 0, 1, 3072))
 
 
 def test_gradient_argument_checks_before_any_gpu_work():

@@ -138,6 +138,48 @@ def test_batch_gradient_matches_single_calls():
     plan.close()
 
 
+def test_batch_every_tile_agrees():
+    """The autotuner's tile dimensions change no output: every (walkers per sample workgroup, 256-point blocks
+    per workgroup) of the forward sample kernel gives identical chi2 (each block's partial is the same
+    fixed-order sum), every LDS window of the gradient sample kernel identical chi2 and gradients to 1e-12
+    relative (only the order of the atomic additions into the half spectrum changes)."""
+    from galario_amd import torch_api as ta
+    sets, _ = make_sets(specs=((256, 3000, 0.02), (512, 20000, 0.01), (1024, 9000, 0.005)))
+    W = 5
+    I, geom, Rmin, dR = walkers(W)
+    plan = ta.BatchedChi2Profile(sets, I.shape[1], Rmin, dR, max_walkers=W, check_central_pixel=False)
+    c0 = plan(I, geom).cpu().numpy()
+    assert plan.tile(W)[0] in (1, 2) and plan.tile(W)[1] == 0
+    chunk = plan.info(W)[0]
+    for kw in (1, 2, 4, 8):
+        for pb in (1, 2):
+            plan.set_chunk(W, chunk, kw)
+            plan.set_tile(W, point_blocks=pb)
+            assert plan.tile(W) == (pb, 0)
+            assert np.array_equal(plan(I, geom).cpu().numpy(), c0), (kw, pb)
+    chi2, _, gI, gg = plan.grad(I, geom)
+    assert np.array_equal(chi2.cpu().numpy(), c0)
+    assert plan.tile(W, grad=True)[1] in (2048, 3072, 4096)
+    for win in (2048, 3072, 4096):
+        plan.set_tile(W, grad_window=win, grad=True)
+        assert plan.tile(W, grad=True) == (1, win)
+        c2, _, gI2, gg2 = plan.grad(I, geom)
+        assert np.array_equal(c2.cpu().numpy(), c0), win
+        assert torch.linalg.norm(gI2 - gI) <= 1e-12 * torch.linalg.norm(gI), win
+        np.testing.assert_allclose(gg2.cpu().numpy(), gg.cpu().numpy(), rtol=1e-12)
+    # a tile fixed before the first call is kept by the timed choice of the chunk
+    plan2 = ta.BatchedChi2Profile(sets, I.shape[1], Rmin, dR, max_walkers=W, check_central_pixel=False)
+    plan2.set_tile(W, point_blocks=2)
+    assert np.array_equal(plan2(I, geom).cpu().numpy(), c0)
+    assert plan2.tile(W) == (2, 0) and plan2.info(W)[0] >= 1
+    with pytest.raises(ValueError, match="point blocks"):
+        plan2.set_tile(W, point_blocks=3)
+    with pytest.raises(ValueError, match="gradient window"):
+        plan2.set_tile(W, grad_window=1000, grad=True)
+    plan.close()
+    plan2.close()
+
+
 def test_batch_single_precision():
     from galario_amd import torch_api as ta
     sets, _ = make_sets(dtype=torch.float32, specs=((256, 3000, 0.02), (1024, 30000, 0.005)))
