@@ -128,6 +128,14 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass_sym_b(SymColArg
     colpass_sym_body<T, LOG2N>(a, blockIdx.x, gridDim.x);
 }
 
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(ColSub<LOG2N>::NT) void k_colpass_sub_b(SymColArgs<T> a, BatchPtrs<T> b) {
+    const int slot = blockIdx.y;
+    a.Ts = b.Ts + slot * b.sTs;
+    a.H = b.H + slot * b.sH;
+    colpass_sub_body<T, LOG2N>(a, xcd_column(blockIdx.x, gridDim.x));
+}
+
 template <typename T>
 __device__ __forceinline__ void walker_sample_args(SampleArgs<T>& a, const BatchPtrs<T>& b, int slot, int w) {
     const WalkerPrm<T> p = b.wp[w];

@@ -354,4 +354,86 @@ struct BlockFft {
     }
 };
 
+// One wave's 512-point transform at the fractional frequencies K + k0/P
+// (K = 0..511): Z[K] = sum_m y[m] W_512^(m (K + k0/P)), 64 lanes x 8
+// elements, BlockFft's three radix-8 Stockham passes at N = 512 with every
+// twiddle exponent q of pass p (q = lane mod 8^p) moved to q + k0/P, i.e.
+// W_N^((P q + k0) 512 / 8^(p+1)) from the N-point table (N = 512 P).  Pass 0's
+// factors W_N^(64 k0 r) are the same for every lane: the caller applies them
+// with the input.  The exchanges go through the wave's own LDS region, real
+// parts then imaginary parts (slots swizzled against bank conflicts), with no
+// workgroup barrier (LDS operations of one wave execute in issue order; the
+// fences keep the compiler from moving them).  Pass twiddles W^e1 come from
+// one table read each (pass 2's as W_512^lane W_N^k0, a contiguous read of
+// the 512-point table), W^2e1 and W^4e1 by squaring.  On entry
+// v[r] = y[lane + 64 r] W_N^(64 k0 r); on return v[r] = Z[lane + 64 r].
+template <typename T>
+struct ResidueFft {
+    using S = FftShape<9>;
+    using B = BlockFft<T, 9, false, true>;
+    static constexpr int E = S::E, TWC = B::TWC;
+    static constexpr int LDS_REALS = S::N;           // per wave
+    static_assert(S::NT == 64 && E == 8 && S::NPASS == 3, "one wave of 8 elements per lane");
+
+    // LDS slot of position i: i ^ ((i >> 3) & 15) keeps the 512 positions in 512 slots and spreads
+    // each exchange instruction's 16-lane groups over 32 distinct 4-byte banks (positions 8 apart,
+    // as pass 0 writes them, land in different banks)
+    static __device__ __forceinline__ int slot(int i) { return i ^ ((i >> 3) & 15); }
+
+    template <int P, int PART>
+    static __device__ __forceinline__ void store_lds(T* lds, const cpx<T> (&v)[E], int lane) {
+        constexpr int NS = S::ns_before(P);
+        const int k = lane & (NS - 1);
+        const int base = (lane - k) * E + k;
+#pragma unroll
+        for (int r = 0; r < E; ++r) lds[slot(base + r * NS)] = PART ? v[r].im : v[r].re;
+    }
+
+    template <int P, int PART>
+    static __device__ __forceinline__ void load_lds(cpx<T> (&v)[E], const T* lds, int lane) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const T x = lds[slot(lane + r * (S::N / E))];
+            if (PART) v[r].im = x;
+            else v[r].re = x;
+        }
+    }
+
+    static __device__ __forceinline__ void wave_sync() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    template <int P>
+    static __device__ __forceinline__ void exchange(cpx<T> (&v)[E], T* lds, int lane) {
+        store_lds<P - 1, 0>(lds, v, lane);
+        wave_sync();
+        load_lds<P, 0>(v, lds, lane);
+        wave_sync();
+        store_lds<P - 1, 1>(lds, v, lane);
+        wave_sync();
+        load_lds<P, 1>(v, lds, lane);
+        wave_sync();
+    }
+
+    static __device__ __forceinline__ void bases(cpx<T> (&tc)[TWC], cpx<T> w) {
+        tc[0] = w;
+        tc[1] = cmul(w, w);
+        tc[2] = cmul(tc[1], tc[1]);
+    }
+
+    static __device__ __forceinline__ void run(cpx<T> (&v)[E], T* lds, int lane, const cpx<T>* __restrict__ tw,
+                                               const cpx<T>* __restrict__ tw512, int P, int k0) {
+        dft<E, false>(v);
+        cpx<T> tc[TWC];
+        bases(tc, tw[(P * (lane & 7) + k0) * 8]);
+        exchange<1>(v, lds, lane);
+        B::template compute<1>(v, tc);
+        bases(tc, cmul(tw512[lane], tw[k0]));
+        exchange<2>(v, lds, lane);
+        B::template compute<2>(v, tc);
+    }
+};
+
 }  // namespace gl

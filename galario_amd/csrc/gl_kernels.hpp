@@ -390,6 +390,7 @@ struct SymColArgs {
     T* H;
     int ldH;
     double rc2;                  // > 0: store only cells within sqrt(rc2) (cells) of the origin (stored_rows)
+    const cpx<T>* tw512;         // W_512^e table (k_colpass_sub)
 };
 
 // Highest row of column l of a half spectrum that a sample within the store
@@ -406,10 +407,9 @@ __device__ __forceinline__ int stored_rows(double rc2, int l, int N) {
 // H[N/2][k] = sum over the loaded rows n of Ts[|n|][N/2] cos(2 pi n k / N), i.e.
 // Ts[0] + 2 sum_{0<m<Rs} Ts[m] cos(2 pi m k/N) + Ts[Rs] cos(2 pi Rs k/N), as a
 // direct sum (FP64 accumulation) by one wave per output k = blk + q N/4.
-template <typename T, int LOG2N>
+template <typename T, int LOG2N, int NW = (FftShape<LOG2N>::NT / 64 > 0 ? FftShape<LOG2N>::NT / 64 : 1)>
 __device__ __forceinline__ void colpass_nyquist(const SymColArgs<T>& a, int blk) {
-    constexpr int N = 1 << LOG2N, NT = FftShape<LOG2N>::NT;
-    constexpr int NW = NT / 64 > 0 ? NT / 64 : 1;
+    constexpr int N = 1 << LOG2N;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int kmax = stored_rows(a.rc2, N / 2, N);
     for (int q = wv; q < 3; q += NW) {
@@ -452,6 +452,88 @@ __device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk
         }
     }
     colpass_nyquist<T, LOG2N>(a, pair);
+}
+
+// Pruned column transforms.  Column pair (la, la + 1) of Ts is nonzero only
+// for signed positions |n'| < Rs (row |n'| of Ts; rows rmax = Rs - 1 and Rs lie
+// beyond the swept profile and hold zeros, unless Rs = N/2).  With Rs <= 256
+// that support fits in the window n' in [-256, 256) of 512 positions, so
+// output k = k0 + P K (P = N / 512) of the N-point transform is the 512-point
+// transform, at K, of z[m] = x[m'] W_N^(m' k0), m' = m or m - 512 the signed
+// position of window slot m.  The input is even, so X[N - k] = X[k]: residues
+// k0 = 0..P/2 give every row 0..N/2 (residue P - k0 as the mirror of residue
+// k0).  One wave per residue (NK = P/2 + 1 waves per workgroup, one workgroup
+// per column pair): 40% fewer FP64 operations per column pair than the full
+// transform and a shorter path per workgroup (no barrier inside the
+// transform), for about the same LDS traffic (each residue wave reads the
+// whole window, and the outputs, rows P apart per wave, are transposed
+// through LDS into contiguous row stores).
+template <int LOG2N>
+struct ColSub {
+    static constexpr int N = 1 << LOG2N, P = N / 512, NK = P / 2 + 1, NT = 64 * NK;
+    static_assert(LOG2N >= 9 && LOG2N <= 13, "pruned column transforms for 512 <= N <= 8192");
+};
+constexpr int COLSUB_MAX_RS = 256;
+
+template <typename T, int LOG2N>
+__device__ __forceinline__ void colpass_sub_body(const SymColArgs<T>& a, int pair) {
+    using CS = ColSub<LOG2N>;
+    using F = ResidueFft<T>;
+    constexpr int N = CS::N, P = CS::P;
+    // LDS: the window (read once per workgroup) and the waves' exchange regions, then the output
+    // rows 0..N/2 (row i at i + i/8: the residue waves write rows P apart, which the padding spreads
+    // over the banks), read back in row order for contiguous stores
+    constexpr size_t FFT_BYTES = 512 * sizeof(cpx<T>) + CS::NK * F::LDS_REALS * sizeof(T);
+    constexpr size_t ROW_BYTES = (N / 2 + N / 16 + 1) * sizeof(cpx<T>);
+    __shared__ __align__(16) unsigned char smem[FFT_BYTES > ROW_BYTES ? FFT_BYTES : ROW_BYTES];
+    cpx<T>* xin = reinterpret_cast<cpx<T>*>(smem);
+    T* lds = reinterpret_cast<T*>(smem + 512 * sizeof(cpx<T>));
+    cpx<T>* rows = reinterpret_cast<cpx<T>*>(smem);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int k0 = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int la = 2 * pair;
+    const int kc = stored_rows(a.rc2, la, N);   // rows a sample can read (column la + 1: no more)
+    if (kc >= 0) {
+        // window slot m holds signed position m' = m (m < 256) or m - 512; P = 1: the whole
+        // column, position -256 = N/2 included
+#pragma unroll
+        for (int q = 0; q < (512 + CS::NT - 1) / CS::NT; ++q) {
+            const int m = tid + q * CS::NT;
+            if (m < 512) {
+                const int mp = m < 256 ? m : m - 512;
+                const bool in = mp >= 0 ? mp < a.Rs : (P == 1 ? -mp <= a.Rs : -mp < a.Rs);
+                const T* src = a.Ts + ts_index(in ? (mp >= 0 ? mp : -mp) : 0, la, a.ldT);
+                const T re = src[0], im = src[1];
+                xin[m] = in ? cpx<T>{re, im} : cpx<T>{T(0), T(0)};
+            }
+        }
+        __syncthreads();
+        // y[m] = x[m'] W_N^(-512 k0 [m' < 0]) with pass 0's W_N^(64 k0 r): one factor per r
+        cpx<T> v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            v[r] = xin[lane + 64 * r];
+            if (r != 0 && k0 != 0) v[r] = cmul(v[r], a.tw[(64 * k0 * (r < 4 ? r : r - 8)) & (N - 1)]);
+        }
+        F::run(v, lds + k0 * F::LDS_REALS, lane, a.tw, a.tw512, P, k0);
+        __syncthreads();                             // every wave is done with the window and its region
+        const bool mirror = k0 != 0 && 2 * k0 != P;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = k0 + P * (lane + 64 * r);
+            const int row = k <= N / 2 ? k : (mirror ? N - k : -1);
+            if (row >= 0 && row <= kc) rows[row + (row >> 3)] = v[r];
+        }
+        __syncthreads();
+        cpx<T>* hab = reinterpret_cast<cpx<T>*>(a.H + h_index(la, 0, a.ldH));   // columns la, la + 1
+        for (int row = tid; row <= kc; row += CS::NT) hab[row] = rows[row + (row >> 3)];
+    }
+    colpass_nyquist<T, LOG2N, CS::NK>(a, pair);
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(ColSub<LOG2N>::NT) void k_colpass_sub(SymColArgs<T> a) {
+    colpass_sub_body<T, LOG2N>(a, xcd_column(blockIdx.x, gridDim.x));
 }
 
 // At most 128 VGPRs (4 waves per SIMD): one call's N/4 workgroups then all fit

@@ -197,6 +197,46 @@ def test_all_double(pars):
     assert math.isclose(c_p, c_ref, rel_tol=1e-12)
 
 
+@pytest.mark.parametrize("nxy", [1024, 2048, 4096, 8192])
+@pytest.mark.parametrize("rmax", [37, 255, 256])
+def test_profile_column_window(nxy, rmax):
+    """The column transform has two forms: with the profile's support inside a 512-row window
+    (rmax <= 255 pixels, N >= 1024) one 512-point transform per residue class of output rows, else
+    the full N-point transform.  Both, at the window's edge (rmax 255 and 256), against the oracle at
+    1e-12 (visibilities relative to max|V|, chi2 relative); float at 1e-5 for rmax 255."""
+    nvis = 2000
+    dxy = 0.01 * arcsec
+    duv = 1 / (nxy * dxy)
+    rng = np.random.default_rng(rmax + nxy)
+    b = rng.uniform(0.5 * duv, 0.45 * nxy * duv, nvis)
+    th = rng.uniform(0, 2 * np.pi, nvis)
+    u, v = b * np.cos(th), b * np.sin(th)
+    nr = 400
+    Rmin = 1e-4 * arcsec
+    dR = (rmax * dxy - Rmin) / nr * (1 - 1e-9)       # ceil((Rmin + nr dR) / dxy) = rmax
+    assert math.ceil((Rmin + nr * dR) / dxy) == rmax
+    R = Rmin + dR * np.arange(nr)
+    I = 1e10 * (np.exp(-((R - 0.3 * R[-1]) / (0.2 * R[-1])) ** 2) + 0.2 * np.exp(-R / R[-1]))
+    inc, PA, dRA, dDec = 0.7, 0.4, 0.02 * arcsec, -0.03 * arcsec
+    ref = O.sample_profile(I, Rmin, dR, dxy, nxy, inc, dRA, dDec, duv, PA, u, v, check_central=False)
+    got = gD.sampleProfile(I, Rmin, dR, nxy, dxy, u, v, inc=inc, PA=PA, dRA=dRA, dDec=dDec,
+                           check_central_pixel=False)
+    assert_vis_close(got, ref, 1e-12)
+    re, im = rng.standard_normal(nvis), rng.standard_normal(nvis)
+    w = rng.uniform(0.5, 2.0, nvis)
+    c_ref = O.reduce_chi2(re, im, ref, w)
+    c = gD.chi2Profile(I, Rmin, dR, nxy, dxy, u, v, re, im, w, inc=inc, PA=PA, dRA=dRA, dDec=dDec,
+                       check_central_pixel=False)
+    assert math.isclose(c, c_ref, rel_tol=1e-12)
+    if rmax == 255 and nxy <= 4096:
+        f = np.float32
+        ref32 = O.sample_profile(I.astype(f), f(Rmin), f(dR), f(dxy), nxy, f(inc), f(dRA), f(dDec), f(duv),
+                                 f(PA), u.astype(f), v.astype(f), check_central=False, dtype=f)
+        got32 = gS.sampleProfile(I.astype(f), f(Rmin), f(dR), nxy, f(dxy), u.astype(f), v.astype(f), inc=f(inc),
+                                 PA=f(PA), dRA=f(dRA), dDec=f(dDec), check_central_pixel=False)
+        assert_vis_close(got32, ref32, 1e-5)
+
+
 def test_all_single():
     """Single precision at 1e-5 against the float32 oracle (par1 of python/test_galario.py)."""
     dt = np.float32

@@ -10,8 +10,11 @@
     relative 1e-6, with galario's kinks handled as in test_gpu_grad.py:
     points whose interpolated spectrum changes sign between the two
     evaluations are dropped, for PA also points whose grid cell changes; the
-    profile direction is smooth; the step is 1e-6 relative except for inc
-    (1e-8).  inc moves every pixel radius: a pixel crossing a knot of the
+    profile direction is smooth; the step is 1e-6 relative for the profile
+    and PA, 1e-5 for dRA and dDec (smooth: they only move each point's
+    phase, and the larger step keeps the two chi2 sums' rounding well below
+    their difference), 1e-8 for inc.  inc moves every pixel radius: a pixel
+    crossing a knot of the
     profile's linear interpolation (the jittered profiles have large slope
     jumps there) gives a central difference an O(h) error, and one crossing
     the last bin (galario's sweep drops pixels beyond it,
@@ -145,8 +148,12 @@ def _fd_cases(wl, I, g):
     dirI = I * (np.sin(R / 37.0) + 0.5 * np.cos(R / 91.0))
     h = 1e-6
     cases = [('profile', I + h * dirI, I - h * dirI, g, g, 2 * h)]
+    # inc: 1e-8 (the nudge above keeps every pixel's bin); PA: 1e-6 (points changing cell are dropped);
+    # dRA, dDec: 1e-5 -- they enter only through the phase 2 pi (u dRA + v dDec) of each point, so chi2 is
+    # smooth in them (truncation error ~ (2 pi u h)^2 / 6 < 1e-8 relative) and the larger step keeps the
+    # rounding of the two chi2 sums (~1e-16 chi2 each) well below the difference
     for j in range(4):
-        hj = (1e-8 if j == 0 else 1e-6) * abs(g[j])
+        hj = {0: 1e-8, 1: 1e-6}.get(j, 1e-5) * abs(g[j])
         gp, gm = g.copy(), g.copy()
         gp[j] += hj
         gm[j] -= hj
@@ -174,10 +181,10 @@ def test_differentiation_workload_vs_finite_differences(diff_workload, k):
         cp = ds.chi2_profile(torch.tensor(Ip, device=DEV), wl.Rmin, wl.dR, wl.nxy, wl.dxy, **kw(gp)).item()
         cm = ds.chi2_profile(torch.tensor(Im, device=DEV), wl.Rmin, wl.dR, wl.nxy, wl.dxy, **kw(gm)).item()
         fd = (cp - cm) / den
-        errs.append((name, abs(fd - an) / abs(an)))
+        errs.append((name, abs(fd - an) / abs(an), abs(cp - cm) / abs(c)))
     print(f"\npoint {k}: {nsign} sign changes, {int((~keep).sum()) - nsign} cell changes dropped; rel errors "
-          + ", ".join(f"{n} {e:.1e}" for n, e in errs))
-    bad = [(n, e) for n, e in errs if not e <= 1e-6]
+          + ", ".join(f"{n} {e:.1e} (|dchi2|/chi2 {d:.0e})" for n, e, d in errs))
+    bad = [(n, e) for n, e, _ in errs if not e <= 1e-6]
     assert not bad, bad
 
 
