@@ -299,6 +299,22 @@ void launch_colpass_L(int blocks, hipStream_t s, const ColArgs<T>& a) {
         default: fail(GALARIO_ERR_RUNTIME, "internal: no FFT kernel for this size"); \
     }
 
+
+// The residue column transforms (k_colpass_sub) when the profile's support
+// fits their 512-row window and N >= 1024 (at N = 512 there is one residue,
+// the whole column, and the workgroup transform is as fast); the full
+// workgroup transform otherwise.
+inline bool residue_fits(int L, int Rs) { return L >= 10 && L <= 13 && Rs <= RESIDUE_MAX_RS; }
+
+#define GL_RESIDUE_SWITCH(CALL)                                                  \
+    switch (L) {                                                                \
+        case 10: CALL(10); break;                                               \
+        case 11: CALL(11); break;                                               \
+        case 12: CALL(12); break;                                               \
+        case 13: CALL(13); break;                                               \
+        default: fail(GALARIO_ERR_RUNTIME, "internal: no residue kernel for this size"); \
+    }
+
 template <typename T>
 void launch_rowpass_sym(int L, int blocks, hipStream_t s, const SymRowArgs<T>& a) {
 #define GL_CALL(LL) hipLaunchKernelGGL((k_rowpass_sym<T, LL>), dim3(blocks), dim3(FftShape<LL>::NT), 0, s, a)
@@ -306,27 +322,11 @@ void launch_rowpass_sym(int L, int blocks, hipStream_t s, const SymRowArgs<T>& a
 #undef GL_CALL
 }
 
-// The pruned column transforms (k_colpass_sub) when the row support fits their
-// 512-position window and N >= 1024 (at N = 512 there is one residue, the
-// whole column, and the workgroup transform is as fast); the full workgroup
-// transform otherwise.
-inline bool colsub_fits(int L, int Rs) { return L >= 10 && L <= 13 && Rs <= COLSUB_MAX_RS; }
-
-#define GL_COLSUB_SWITCH(CALL)                                                  \
-    switch (L) {                                                                \
-        case 9: CALL(9); break;                                                 \
-        case 10: CALL(10); break;                                               \
-        case 11: CALL(11); break;                                               \
-        case 12: CALL(12); break;                                               \
-        case 13: CALL(13); break;                                               \
-        default: fail(GALARIO_ERR_RUNTIME, "internal: no pruned column kernel for this size"); \
-    }
-
 template <typename T>
 void launch_colpass_sym(int L, int blocks, hipStream_t s, const SymColArgs<T>& a) {
-    if (colsub_fits(L, a.Rs)) {
-#define GL_CALL(LL) hipLaunchKernelGGL((k_colpass_sub<T, LL>), dim3(blocks), dim3(ColSub<LL>::NT), 0, s, a)
-        GL_COLSUB_SWITCH(GL_CALL)
+    if (residue_fits(L, a.Rs)) {
+#define GL_CALL(LL) hipLaunchKernelGGL((k_colpass_sub<T, LL>), dim3(blocks), dim3(ResidueShape<LL>::NT), 0, s, a)
+        GL_RESIDUE_SWITCH(GL_CALL)
 #undef GL_CALL
         return;
     }
@@ -482,7 +482,7 @@ bool build_spectrum_sym(Workspace& w, hipStream_t s, const SweepPrm<T>& sw, int 
     T* H = w.S.get<T>(size_t(g.C + 1) * g.ldH);
     const cpx<T>* tw = twiddles<T>(N);
     SymRowArgs<T> ra{sw, N, g.Rs, g.nrows, g.ldT, tw, Ts, zero, nzero, zero1};
-    SymColArgs<T> ca{Ts, N, g.Rs, g.ldT, tw, H, g.ldH, rc2, colsub_fits(L, g.Rs) ? twiddles<T>(512) : nullptr};
+    SymColArgs<T> ca{Ts, N, g.Rs, g.ldT, tw, H, g.ldH, rc2, residue_fits(L, g.Rs) ? twiddles<T>(512) : nullptr};
     t_timer.mark(ST_ROW, s);
     for (int r = t_timer.reps(ST_ROW); r > 0; --r) launch_rowpass_sym<T>(L, g.pairs_r, s, ra);
     t_timer.mark(ST_COL, s);
@@ -1350,16 +1350,16 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, const BatchShape& sh, 
             SymRowArgs<T> ra{G.sw, G.N, g.Rs, g.nrows, g.ldT, tw, nullptr, nullptr, 0, nullptr};
             SymColArgs<T> ca{nullptr, G.N, g.Rs,     g.ldT,
                              tw,      nullptr, g.ldH, G.ex.rc2,
-                             colsub_fits(L, g.Rs) ? twiddles<T>(512) : nullptr};
+                             residue_fits(L, g.Rs) ? twiddles<T>(512) : nullptr};
             const int cblocks = G.N / 4;                      // column pairs 0..N/2-1, column N/2 inside
             const int ablocks = 8 * ((g.pairs_c + 7) / 8);    // adjoint: (C+1)/2 pairs
 #define GL_CALL(LL) \
     hipLaunchKernelGGL((k_rowpass_sym_b<T, LL>), dim3(g.pairs_r, nw), dim3(FftShape<LL>::NT), 0, s, ra, b)
             GL_L_SWITCH(T, GL_CALL)
 #undef GL_CALL
-            if (colsub_fits(L, g.Rs)) {
-#define GL_CALL(LL) hipLaunchKernelGGL((k_colpass_sub_b<T, LL>), dim3(cblocks, nw), dim3(ColSub<LL>::NT), 0, s, ca, b)
-                GL_COLSUB_SWITCH(GL_CALL)
+            if (residue_fits(L, g.Rs)) {
+#define GL_CALL(LL) hipLaunchKernelGGL((k_colpass_sub_b<T, LL>), dim3(cblocks, nw), dim3(ResidueShape<LL>::NT), 0, s, ca, b)
+                GL_RESIDUE_SWITCH(GL_CALL)
 #undef GL_CALL
             } else {
 #define GL_CALL(LL) \
@@ -1751,7 +1751,7 @@ int chi2_profile_noalloc_dev(void* stream, int nr, const T* I, T Rmin, T dR, T d
         hipStream_t s = static_cast<hipStream_t>(stream);
         const SymRowArgs<T> ra{ps.sw, nxy, g.Rs, g.nrows, g.ldT, tw, Ts, nullptr, 0, nullptr};
         const SymColArgs<T> ca{Ts, nxy, g.Rs, g.ldT, tw, H, g.ldH, ex.rc2,
-                               colsub_fits(L, g.Rs) ? twiddles_built<T>(dev, 512) : nullptr};
+                               residue_fits(L, g.Rs) ? twiddles_built<T>(dev, 512) : nullptr};
         launch_rowpass_sym<T>(L, g.pairs_r, s, ra);
         launch_colpass_sym<T>(L, nxy / 4, s, ca);
         SampleArgs<T> sa = sample_args<T>(nullptr, 0, 0, nxy, g.C, T(1), duv, ss, nd, u, v);

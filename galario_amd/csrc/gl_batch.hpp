@@ -129,7 +129,7 @@ __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_colpass_sym_b(SymColArg
 }
 
 template <typename T, int LOG2N>
-__global__ __launch_bounds__(ColSub<LOG2N>::NT) void k_colpass_sub_b(SymColArgs<T> a, BatchPtrs<T> b) {
+__global__ __launch_bounds__(ResidueShape<LOG2N>::NT) void k_colpass_sub_b(SymColArgs<T> a, BatchPtrs<T> b) {
     const int slot = blockIdx.y;
     a.Ts = b.Ts + slot * b.sTs;
     a.H = b.H + slot * b.sH;

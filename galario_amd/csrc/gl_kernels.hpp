@@ -469,15 +469,15 @@ __device__ __forceinline__ void colpass_sym_body(const SymColArgs<T>& a, int blk
 // whole window, and the outputs, rows P apart per wave, are transposed
 // through LDS into contiguous row stores).
 template <int LOG2N>
-struct ColSub {
+struct ResidueShape {
     static constexpr int N = 1 << LOG2N, P = N / 512, NK = P / 2 + 1, NT = 64 * NK;
-    static_assert(LOG2N >= 9 && LOG2N <= 13, "pruned column transforms for 512 <= N <= 8192");
+    static_assert(LOG2N >= 9 && LOG2N <= 13, "residue transforms for 512 <= N <= 8192");
 };
-constexpr int COLSUB_MAX_RS = 256;
+constexpr int RESIDUE_MAX_RS = 256;
 
 template <typename T, int LOG2N>
 __device__ __forceinline__ void colpass_sub_body(const SymColArgs<T>& a, int pair) {
-    using CS = ColSub<LOG2N>;
+    using CS = ResidueShape<LOG2N>;
     using F = ResidueFft<T>;
     constexpr int N = CS::N, P = CS::P;
     // LDS: the window (read once per workgroup) and the waves' exchange regions, then the output
@@ -532,7 +532,7 @@ __device__ __forceinline__ void colpass_sub_body(const SymColArgs<T>& a, int pai
 }
 
 template <typename T, int LOG2N>
-__global__ __launch_bounds__(ColSub<LOG2N>::NT) void k_colpass_sub(SymColArgs<T> a) {
+__global__ __launch_bounds__(ResidueShape<LOG2N>::NT) void k_colpass_sub(SymColArgs<T> a) {
     colpass_sub_body<T, LOG2N>(a, xcd_column(blockIdx.x, gridDim.x));
 }
 
