@@ -91,7 +91,7 @@ def frac_of(nbytes, evals_per_s):
     return round(nbytes * evals_per_s / (HBM_PEAK_GBS * 1e9), 4)
 
 
-KERNEL_NAMES = {'row_pass': 'k_rowpass_sym<double, 12>', 'column_pass': 'k_colpass_sym<double, 12>',
+KERNEL_NAMES = {'row_pass': 'k_rowpass_sym<double, 12>', 'column_pass': 'k_colpass_sub<double, 12>',
                 'sample': 'k_sample_chi2_n<double, true, 2>'}
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hbm_traffic_profile_nxy4096_nvis1e6_f64.txt')
 
