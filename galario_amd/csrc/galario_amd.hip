@@ -113,7 +113,12 @@ struct DevBuf {
         if (b < 256) b = 256;
         if (b > cap) {
             release();
-            HC(hipMalloc(&p, b));
+            const hipError_t e = hipMalloc(&p, b);
+            if (e != hipSuccess) {
+                p = nullptr;
+                (void)hipGetLastError();   // a failed allocation must not surface in a later launch check
+                HC(e);                     // out of memory: std::bad_alloc (MemoryError in Python)
+            }
             cap = b;
         }
         return static_cast<X*>(p);

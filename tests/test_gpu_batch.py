@@ -180,6 +180,21 @@ def test_batch_every_tile_agrees():
     plan2.close()
 
 
+def test_allocation_failure_raises_memory_error():
+    """galario maps std::bad_alloc to MemoryError (src/galario.cpp:158-198, docs/py-api.rst:72-82): a plan
+    whose parameter buffers exceed the GPU's memory (2^31 - 1 walkers x 300 bins: 5 TB) fails with
+    MemoryError, and the next plan and call work normally (the failed allocation leaves no error behind)."""
+    from galario_amd import torch_api as ta
+    sets, _ = make_sets(specs=((256, 3000, 0.02),))
+    I, geom, Rmin, dR = walkers(2)
+    with pytest.raises(MemoryError):
+        ta.BatchedChi2Profile(sets, I.shape[1], Rmin, dR, max_walkers=2 ** 31 - 1, check_central_pixel=False)
+    plan = ta.BatchedChi2Profile(sets, I.shape[1], Rmin, dR, max_walkers=2, check_central_pixel=False)
+    ref = np.array(plain(sets, I, geom, Rmin, dR)).sum(axis=1)
+    np.testing.assert_allclose(plan(I, geom).cpu().numpy(), ref, rtol=1e-12)
+    plan.close()
+
+
 def test_batch_single_precision():
     from galario_amd import torch_api as ta
     sets, _ = make_sets(dtype=torch.float32, specs=((256, 3000, 0.02), (1024, 30000, 0.005)))
