@@ -385,7 +385,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=200)
-    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=50)
     ap.add_argument('--cpu-seconds', type=float, default=12.0, help='CPU-baseline time budget (0: skip)')
     ap.add_argument('--nvis', type=int, default=1_000_000)
     ap.add_argument('--nxy', type=int, default=4096)
@@ -453,9 +453,10 @@ def main():
 
     def run_timed(fn, steps, blocks=10):
         """Exactly `steps` calls between events on the kernels' stream, bracketed by synchronisations,
-        in `blocks` blocks of (nearly) equal size: (mean ms per call over all steps, std, min and median
-        over the blocks' per-call means, steps)."""
-        blocks = max(1, min(blocks, steps))
+        in at most `blocks` blocks of (nearly) equal size and at least 50 calls each (a timing event
+        between blocks costs the stream a few microseconds): (mean ms per call over all steps, std, min
+        and median over the blocks' per-call means, steps)."""
+        blocks = max(1, min(blocks, steps // 50))
         sizes = [steps // blocks + (1 if b < steps % blocks else 0) for b in range(blocks)]
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(blocks + 1)]
         torch.cuda.synchronize()
