@@ -342,6 +342,12 @@ void launch_colpass_sym(int L, int blocks, hipStream_t s, const SymColArgs<T>& a
 
 template <typename T>
 void launch_adjcol_sym(int L, int blocks, hipStream_t s, const AdjColArgs<T>& a) {
+    if (residue_fits(L, a.Rs)) {
+#define GL_CALL(LL) hipLaunchKernelGGL((k_adjcol_sub<T, LL>), dim3(blocks), dim3(ResidueShape<LL>::NT), 0, s, a)
+        GL_RESIDUE_SWITCH(GL_CALL)
+#undef GL_CALL
+        return;
+    }
 #define GL_CALL(LL) hipLaunchKernelGGL((k_adjcol_sym<T, LL>), dim3(blocks), dim3(FftShape<LL>::NT), 0, s, a)
     GL_L_SWITCH(T, GL_CALL)
 #undef GL_CALL
@@ -1124,7 +1130,9 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         const cpx<T>* tw = twiddles<T>(nxy);
         const Dst4 dst{{chi2_d, grad_geom + 1, grad_geom + 2, grad_geom + 3}};
         const int nfft = 8 * ((g.pairs_c + 7) / 8);
-        AdjColArgs<T> aca{Hh, hot, g.ldH, nxy, g.Rs, tw, Th, ldTh, part, int(nb), dst, nfft, ex.rc2};
+        AdjColArgs<T> aca{Hh,   hot,  g.ldH, nxy,  g.Rs,    tw,
+                          Th,   ldTh, part,  int(nb), dst, nfft,
+                          ex.rc2, residue_fits(L, g.Rs) ? twiddles<T>(512) : nullptr};
         t_timer.mark(ST_ADJCOL, s);
         launch_adjcol_sym<T>(L, nfft + 8, s, aca);   // + 8 workgroups for the partial sums
         HC(hipGetLastError());
@@ -1402,6 +1410,8 @@ void batch_enqueue(BatchPlan<T>& P, hipStream_t s, int W, const BatchShape& sh, 
                 else if (!mix) launch_sample_multi<T, false>(sh.kw, sh.ppt, nb, nw, s, a, b);
             }
             if (grad) {
+                // the full adjoint column transform: with many walkers in flight the batched passes are
+                // bound by throughput, where the residue form's extra LDS traffic costs ~1-2%
                 AdjColArgs<T> aca{nullptr, nullptr, g.ldH, G.N, g.Rs, tw, nullptr, 0, nullptr, 0, Dst4{}, ablocks,
                                   G.ex.rc2};
                 aca.ldTh = int(P.sTh / size_t(g.nrows));

@@ -258,27 +258,32 @@ struct AdjColArgs {
     Dst4 dst;
     int nfft;
     double rc2;          // > 0: Hh is zero beyond the store radius sqrt(rc2) (stored_rows)
+    const cpx<T>* tw512; // W_512^e table (k_adjcol_sub)
 };
 
 // Deterministic sum *dst = sum_i p[i] of n partials in the order of
 // k_sum_partials (1024 virtual threads, thread t summing p[t], p[t + 1024],
 // ...; the 16 virtual waves' sums added in order), so chi2 equals the forward
-// call's bit for bit; run by one workgroup of NT threads, NT a multiple of 64
-// dividing 1024.
+// call's bit for bit; run by one workgroup of NT threads, NT a multiple of 64:
+// real wave w plays virtual waves w, w + NT/64, ... (below 16), lane for lane.
 template <int NT>
 __device__ __forceinline__ void sum_partials_in_block(const double* __restrict__ p, int n, double* dst,
                                                       double* red16) {
-    constexpr int V = 1024 / NT;              // virtual threads per thread
+    static_assert(NT % 64 == 0 && NT <= 1024, "whole waves");
+    constexpr int NW = NT / 64;
+    constexpr int V = (16 + NW - 1) / NW;     // virtual waves per real wave (the last ones may be beyond 16)
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     double acc[V];
 #pragma unroll
     for (int q = 0; q < V; ++q) {
-        acc[q] = sum_stride1024(p, t + q * NT, n, 0.0);
+        const int vw = wv + q * NW;
+        acc[q] = vw < 16 ? sum_stride1024(p, vw * 64 + lane, n, 0.0) : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) {
+        const int vw = wv + q * NW;
         const double x = wave_sum(acc[q]);
-        if (lane == 0) red16[wv + q * (NT / 64)] = x;
+        if (vw < 16 && lane == 0) red16[vw] = x;
     }
     __syncthreads();
     if (t == 0) {
@@ -301,7 +306,6 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
     constexpr int C = N / 2 + 1;
     constexpr int NPAIR = (C + 1) / 2;
     __shared__ T lds[S::LDS_ELEMS];
-    static_assert(NT >= 64 && 1024 % NT == 0, "partial sums replay 1024 virtual threads");
     if (PARTIALS && blk >= a.nfft) {        // extra workgroups: the partial sums of the sample kernel
         const int j = blk - a.nfft;
         __shared__ double red16[16];
@@ -366,6 +370,146 @@ __device__ __forceinline__ void adjcol_sym_body(const AdjColArgs<T>& a, int blk,
 template <typename T, int LOG2N>
 __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjcol_sym(AdjColArgs<T> a) {
     adjcol_sym_body<T, LOG2N>(a, blockIdx.x, gridDim.x);
+}
+
+// Residue form of the adjoint column pass (N >= 1024, profile within 255 pixels: Rs' <= 256; the single
+// call's: a shorter path per workgroup and 84 instead of 166 VGPRs, so all of one call's workgroups are
+// resident at once, +2-3% on the gradient call; the batched form, bound by throughput, keeps the full
+// transform).  Its
+// outputs are rows m = 0..Rs' of the N-point transform of the even extension z[k] = w_k x[fold(k)]
+// of a column pair of Hh.  With Y_k0 the 512-point transform of z[k0 + P K] (P = N / 512),
+//   Z[m] = sum over k0 = 0..P/2 of  W_N^(m k0) Y_k0[m]  +  W_N^(-m k0) Y_k0[512 - m],
+// the second term for k0 other than 0 and P/2: it is residue P - k0, whose sequence is residue k0's
+// reversed (z is even).  One wave per residue k0: its 512-point transform (ResidueFft at k0 = 0) leaves
+// Y[lane + 64 s] in v[s], and Y[512 - m] of m = lane + 64 s sits in lane 64 - lane, slot 7 - s (lane 0:
+// its own slot 8 - s).  The column pair is staged in LDS, real parts then imaginary parts, for the
+// waves' strided gathers (row p at p + p/8: rows P apart fall in different banks) and cleared as soon
+// as it is read; the waves' terms are summed in LDS in residue order.
+template <typename T, int LOG2N>
+__device__ __forceinline__ void adjcol_sub_body(const AdjColArgs<T>& a, int blk) {
+    using CS = ResidueShape<LOG2N>;
+    using F = ResidueFft<T>;
+    constexpr int N = CS::N, P = CS::P, NT = CS::NT, NK = CS::NK;
+    constexpr int C = N / 2 + 1, NPAIR = (C + 1) / 2;
+    constexpr int PER = (C + NT - 1) / NT;              // staged rows per thread
+    constexpr int MOUT = 260;                           // output rows 0..Rs' (<= 256) per residue
+    constexpr size_t STG_BYTES = size_t(N / 2 + N / 16 + 1) * sizeof(double);
+    constexpr size_t EXCH_BYTES = size_t(NK) * F::LDS_REALS * sizeof(T);
+    constexpr size_t SUM_BYTES = size_t(NK) * MOUT * sizeof(cpx<T>);
+    constexpr size_t BYTES = STG_BYTES + EXCH_BYTES > SUM_BYTES ? STG_BYTES + EXCH_BYTES : SUM_BYTES;
+    __shared__ __align__(16) unsigned char smem[BYTES];
+    double* stg = reinterpret_cast<double*>(smem);
+    T* exch = reinterpret_cast<T*>(smem + STG_BYTES);
+    cpx<T>* sums = reinterpret_cast<cpx<T>*>(smem);
+    const int tid = threadIdx.x;
+    if (blk >= a.nfft) {               // extra workgroups: the partial sums of the sample kernel
+        const int j = blk - a.nfft;
+        __shared__ double red16[16];
+        if (a.part && j < 4) sum_partials_in_block<NT>(a.part + size_t(j) * a.np, a.np, a.dst.d[j], red16);
+        return;
+    }
+    const int pair = xcd_column(blk, a.nfft);
+    if (pair >= NPAIR) return;
+    const int la = 2 * pair, lb = la + 1;
+    const bool has_b = lb < C;
+    const int lane = tid & 63;
+    const int k0 = __builtin_amdgcn_readfirstlane(tid >> 6);
+    cpx<double>* hab = reinterpret_cast<cpx<double>*>(a.Hh + h_index(la, 0, a.ldH));   // columns la, lb
+    const bool hot_cols = la < GHOT;   // GHOT is even: lb < GHOT too
+    const int kc = stored_rows(a.rc2, la, N);
+    if (kc < 0) {                      // no sample reaches these columns: their adjoint rows are zero
+        for (int m = tid; m <= a.Rs; m += NT) {
+            T* o = a.Th + size_t(m) * a.ldTh + la;
+            o[0] = T(0);
+            if (has_b) o[1] = T(0);
+        }
+        return;
+    }
+    // rows 0..kc of the pair (+ the hot tile's replicas), each read by one thread and then cleared
+    cpx<double> xs[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int p = tid + q * NT;
+        xs[q] = cpx<double>{0.0, 0.0};
+        if (p <= kc) {
+            xs[q] = hab[p];
+            if (hot_cols && p < GHOT) {
+                double sa = 0.0, sb = 0.0;
+                for (int r = 0; r < GREP; ++r) {
+                    sa += a.hot[(r * GHOT + la) * GHOT + p];
+                    sb += a.hot[(r * GHOT + lb) * GHOT + p];
+                }
+                xs[q].re += sa;
+                if (has_b) xs[q].im += sb;
+            }
+            hab[p] = cpx<double>{0.0, 0.0};
+        }
+        if (p < C) stg[p + (p >> 3)] = xs[q].re;
+    }
+    __syncthreads();                   // (every read of the hot replicas is done)
+    if (hot_cols)
+        for (int j = tid; j < GREP * GHOT; j += NT) {
+            const int r = j / GHOT, p = j % GHOT;
+            a.hot[(r * GHOT + la) * GHOT + p] = 0.0;
+            if (has_b) a.hot[(r * GHOT + lb) * GHOT + p] = 0.0;
+        }
+    cpx<T> v[8];
+    int sp[8];
+    double wk[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int k = k0 + P * (lane + 64 * r);
+        const int p = k <= N / 2 ? k : N - k;
+        sp[r] = p + (p >> 3);
+        wk[r] = ext_weight(k, N);
+        v[r].re = T(wk[r] * stg[sp[r]]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int p = tid + q * NT;
+        if (p < C) stg[p + (p >> 3)] = has_b ? xs[q].im : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r].im = T(wk[r] * stg[sp[r]]);
+    F::run(v, exch + k0 * F::LDS_REALS, lane, a.tw, a.tw512, P, 0);
+    // this residue's terms of rows m = lane + 64 s
+    const bool mirror = k0 != 0 && 2 * k0 != P;
+    const int src = (64 - lane) & 63;
+    cpx<T> c[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        const int m = lane + 64 * s;
+        const cpx<T> w = a.tw[(m * k0) & (N - 1)];
+        c[s] = cmul(v[s], w);
+        if (mirror) {
+            const T yr = __shfl(v[7 - s].re, src, 64), yi = __shfl(v[7 - s].im, src, 64);
+            const cpx<T> ym = lane == 0 ? v[(8 - s) & 7] : cpx<T>{yr, yi};
+            c[s] = cadd(c[s], cmul(ym, cconj(w)));
+        }
+    }
+    __syncthreads();                   // every wave is done with the staged pair and its exchange region
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        const int m = lane + 64 * s;
+        if (m <= a.Rs) sums[k0 * MOUT + m] = c[s];
+    }
+    __syncthreads();
+    for (int m = tid; m <= a.Rs; m += NT) {
+        cpx<T> acc = sums[m];
+#pragma unroll
+        for (int k = 1; k < NK; ++k) acc = cadd(acc, sums[k * MOUT + m]);
+        const T cm = T(mult(m, N));
+        T* o = a.Th + size_t(m) * a.ldTh + la;
+        o[0] = cm * acc.re;
+        if (has_b) o[1] = cm * acc.im;
+    }
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(ResidueShape<LOG2N>::NT) void k_adjcol_sub(AdjColArgs<T> a) {
+    adjcol_sub_body<T, LOG2N>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
