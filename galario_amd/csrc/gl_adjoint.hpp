@@ -164,14 +164,33 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
     for (int j = tid; j < nwin; j += 256) win[j] = 0.0;
     __syncthreads();
     if (use_win) {
-        if (live) {
-            const int cu[4] = {cu0, cu0, cu1, cu1}, cm[4] = {cm0, cm1, cm0, cm1};
+        // LDS additions, corner by corner.  Neighbouring lanes (points in locality order) often add to the
+        // same cell: lanes l and l ^ 1, then l and l ^ 2, with the same cell merge their values into the
+        // lower lane first (DPP moves), which halves to quarters the same-address LDS atomics where the
+        // points are dense.  key: window offset, or 2^30 + hot-tile offset; -1: nothing to add.
+        const int cu[4] = {cu0, cu0, cu1, cu1}, cm[4] = {cm0, cm1, cm0, cm1};
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (gk[k] == 0.0) continue;
-                if (cu[k] < GHOT && cm[k] < GHOT) lds_add_f64(&hot[cu[k] * GHOT + cm[k]], gk[k]);
-                else lds_add_f64(&win[((cu[k] >> 1) - pmin) * w2 + 2 * (cm[k] - mmin) + (cu[k] & 1)], gk[k]);
+        for (int k = 0; k < 4; ++k) {
+            const bool hotc = cu[k] < GHOT && cm[k] < GHOT;
+            int key = (live && gk[k] != 0.0)
+                          ? (hotc ? (1 << 30) + cu[k] * GHOT + cm[k]
+                                  : ((cu[k] >> 1) - pmin) * w2 + 2 * (cm[k] - mmin) + (cu[k] & 1))
+                          : -1;
+            double val = gk[k];
+            int ko = dpp_i32<DPP_XOR1>(key);
+            double vo = dpp_f64<DPP_XOR1>(val);
+            if (key >= 0 && key == ko) {
+                if (lane & 1) key = -1;
+                else val += vo;
             }
+            ko = dpp_i32<DPP_XOR2>(key);
+            vo = dpp_f64<DPP_XOR2>(val);
+            if (key >= 0 && key == ko) {
+                if (lane & 2) key = -1;
+                else val += vo;
+            }
+            if (key >= (1 << 30)) lds_add_f64(&hot[key - (1 << 30)], val);
+            else if (key >= 0) lds_add_f64(&win[key], val);
         }
     } else {
         // Straight to Hh.  The FP64 atomics are served per 64-byte line and wave instruction, and
