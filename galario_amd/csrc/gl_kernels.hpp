@@ -475,16 +475,18 @@ struct ResidueShape {
 };
 constexpr int RESIDUE_MAX_RS = 256;
 
-template <typename T, int LOG2N>
+template <typename T, int LOG2N, bool SPLITROW = false>
 __device__ __forceinline__ void colpass_sub_body(const SymColArgs<T>& a, int pair) {
     using CS = ResidueShape<LOG2N>;
     using F = ResidueFft<T>;
     constexpr int N = CS::N, P = CS::P;
     // LDS: the window (read once per workgroup) and the waves' exchange regions, then the output
     // rows 0..N/2 (row i at i + i/8: the residue waves write rows P apart, which the padding spreads
-    // over the banks), read back in row order for contiguous stores
+    // over the banks), read back in row order for contiguous stores.  SPLITROW (the batched form, bound
+    // by throughput): the rows pass through LDS real parts first, then imaginary parts, in half the
+    // space, so more workgroups fit on a CU
     constexpr size_t FFT_BYTES = 512 * sizeof(cpx<T>) + CS::NK * F::LDS_REALS * sizeof(T);
-    constexpr size_t ROW_BYTES = (N / 2 + N / 16 + 1) * sizeof(cpx<T>);
+    constexpr size_t ROW_BYTES = (N / 2 + N / 16 + 1) * (SPLITROW ? sizeof(T) : sizeof(cpx<T>));
     __shared__ __align__(16) unsigned char smem[FFT_BYTES > ROW_BYTES ? FFT_BYTES : ROW_BYTES];
     cpx<T>* xin = reinterpret_cast<cpx<T>*>(smem);
     T* lds = reinterpret_cast<T*>(smem + 512 * sizeof(cpx<T>));
@@ -518,15 +520,46 @@ __device__ __forceinline__ void colpass_sub_body(const SymColArgs<T>& a, int pai
         F::run(v, lds + k0 * F::LDS_REALS, lane, a.tw, a.tw512, P, k0);
         __syncthreads();                             // every wave is done with the window and its region
         const bool mirror = k0 != 0 && 2 * k0 != P;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int k = k0 + P * (lane + 64 * r);
-            const int row = k <= N / 2 ? k : (mirror ? N - k : -1);
-            if (row >= 0 && row <= kc) rows[row + (row >> 3)] = v[r];
-        }
-        __syncthreads();
         cpx<T>* hab = reinterpret_cast<cpx<T>*>(a.H + h_index(la, 0, a.ldH));   // columns la, la + 1
-        for (int row = tid; row <= kc; row += CS::NT) hab[row] = rows[row + (row >> 3)];
+        if constexpr (SPLITROW) {
+            constexpr int PER = (N / 2 + 1 + CS::NT - 1) / CS::NT;
+            T* rr = reinterpret_cast<T*>(smem);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int k = k0 + P * (lane + 64 * r);
+                const int row = k <= N / 2 ? k : (mirror ? N - k : -1);
+                if (row >= 0 && row <= kc) rr[row + (row >> 3)] = v[r].re;
+            }
+            __syncthreads();
+            T keep[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int row = tid + q * CS::NT;
+                keep[q] = row <= kc ? rr[row + (row >> 3)] : T(0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int k = k0 + P * (lane + 64 * r);
+                const int row = k <= N / 2 ? k : (mirror ? N - k : -1);
+                if (row >= 0 && row <= kc) rr[row + (row >> 3)] = v[r].im;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int row = tid + q * CS::NT;
+                if (row <= kc) hab[row] = cpx<T>{keep[q], rr[row + (row >> 3)]};
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int k = k0 + P * (lane + 64 * r);
+                const int row = k <= N / 2 ? k : (mirror ? N - k : -1);
+                if (row >= 0 && row <= kc) rows[row + (row >> 3)] = v[r];
+            }
+            __syncthreads();
+            for (int row = tid; row <= kc; row += CS::NT) hab[row] = rows[row + (row >> 3)];
+        }
     }
     colpass_nyquist<T, LOG2N, CS::NK>(a, pair);
 }
