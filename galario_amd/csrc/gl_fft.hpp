@@ -361,24 +361,30 @@ struct BlockFft {
 // W_N^((P q + k0) 512 / 8^(p+1)) from the N-point table (N = 512 P).  Pass 0's
 // factors W_N^(64 k0 r) are the same for every lane: the caller applies them
 // with the input.  The exchanges go through the wave's own LDS region, real
-// parts then imaginary parts (slots swizzled against bank conflicts), with no
+// parts then imaginary parts (slot layout against bank conflicts: slot()), with no
 // workgroup barrier (LDS operations of one wave execute in issue order; the
 // fences keep the compiler from moving them).  Pass twiddles W^e1 come from
 // one table read each (pass 2's as W_512^lane W_N^k0, a contiguous read of
 // the 512-point table), W^2e1 and W^4e1 by squaring.  On entry
 // v[r] = y[lane + 64 r] W_N^(64 k0 r); on return v[r] = Z[lane + 64 r].
-template <typename T>
+// PAD selects the LDS layout of the exchanges (below).
+template <typename T, bool PAD = false>
 struct ResidueFft {
     using S = FftShape<9>;
     using B = BlockFft<T, 9, false, true>;
     static constexpr int E = S::E, TWC = B::TWC;
-    static constexpr int LDS_REALS = S::N;           // per wave
+    static constexpr int LDS_REALS = PAD ? S::N + S::N / 8 : S::N;   // per wave
     static_assert(S::NT == 64 && E == 8 && S::NPASS == 3, "one wave of 8 elements per lane");
 
-    // LDS slot of position i: i ^ ((i >> 3) & 15) keeps the 512 positions in 512 slots and spreads
-    // each exchange instruction's 16-lane groups over 32 distinct 4-byte banks (positions 8 apart,
-    // as pass 0 writes them, land in different banks)
-    static __device__ __forceinline__ int slot(int i) { return i ^ ((i >> 3) & 15); }
+    // LDS slot of position i.  PAD = false: i ^ ((i >> 3) & 15) keeps the 512 positions in 512 slots and
+    // spreads each exchange instruction's 16-lane groups over 32 distinct 4-byte banks (positions 8
+    // apart, as pass 0 writes them, land in different banks).  PAD = true: i + (i >> 3) in 576 slots, a
+    // few bank conflicts more, but every exchange address is a per-lane base plus a constant (positions
+    // base + r, base + 8 r and lane + 64 r go to slots base' + r, base' + 9 r and lane' + 72 r), so the
+    // offsets fold into the LDS instructions: the column pass's code has 33 fewer VALU and 16 fewer LDS
+    // instructions, and its workgroups fit on a CU as before (its LDS is sized by the output transpose).  The adjoint column pass
+    // keeps the 512-slot form, where 576 slots would cost a workgroup per CU.
+    static __device__ __forceinline__ int slot(int i) { return PAD ? i + (i >> 3) : i ^ ((i >> 3) & 15); }
 
     template <int P, int PART>
     static __device__ __forceinline__ void store_lds(T* lds, const cpx<T> (&v)[E], int lane) {

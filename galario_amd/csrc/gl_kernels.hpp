@@ -478,7 +478,7 @@ constexpr int RESIDUE_MAX_RS = 256;
 template <typename T, int LOG2N, bool SPLITROW = false>
 __device__ __forceinline__ void colpass_sub_body(const SymColArgs<T>& a, int pair) {
     using CS = ResidueShape<LOG2N>;
-    using F = ResidueFft<T>;
+    using F = ResidueFft<T, true>;
     constexpr int N = CS::N, P = CS::P;
     // LDS: the window (read once per workgroup) and the waves' exchange regions, then the output
     // rows 0..N/2 (row i at i + i/8: the residue waves write rows P apart, which the padding spreads
