@@ -355,6 +355,12 @@ void launch_adjcol_sym(int L, int blocks, hipStream_t s, const AdjColArgs<T>& a)
 
 template <typename T>
 void launch_adjrow_sym(int L, int blocks, hipStream_t s, const AdjRowArgs<T>& a) {
+    if (a.tw512 && residue_fits(L, a.Rs)) {
+#define GL_CALL(LL) hipLaunchKernelGGL((k_adjrow_sub<T, LL>), dim3(blocks), dim3(ResidueShape<LL>::NT), 0, s, a)
+        GL_RESIDUE_SWITCH(GL_CALL)
+#undef GL_CALL
+        return;
+    }
 #define GL_CALL(LL) hipLaunchKernelGGL((k_adjrow_sym<T, LL>), dim3(blocks), dim3(FftShape<LL>::NT), 0, s, a)
     GL_L_SWITCH(T, GL_CALL)
 #undef GL_CALL
@@ -1137,7 +1143,8 @@ int chi2_profile_grad_dev(void* stream, int nr, const T* I, T Rmin, T dR, T dxy,
         launch_adjcol_sym<T>(L, nfft + 8, s, aca);   // + 8 workgroups for the partial sums
         HC(hipGetLastError());
         w.Hh_zero = true;
-        AdjRowArgs<T> ara{ps.sw, Th, ldTh, nxy, g.Rs, g.nrows, tw, std::tan(double(inc)), grad_I, grad_geom};
+        AdjRowArgs<T> ara{ps.sw,  Th,        ldTh,     nxy, g.Rs, g.nrows, tw, std::tan(double(inc)),
+                          grad_I, grad_geom, aca.tw512};   // tw512 set: the residue form where it fits
         t_timer.mark(ST_ADJROW, s);
         launch_adjrow_sym<T>(L, g.pairs_r, s, ara);
         HC(hipGetLastError());

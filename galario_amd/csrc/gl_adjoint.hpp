@@ -545,6 +545,7 @@ struct AdjRowArgs {
     double tan_inc;
     double* gI;          // nr, zeroed by the forward row pass
     double* ginc;        // 1, zeroed by the forward row pass
+    const cpx<T>* tw512; // W_512^e table (k_adjrow_sub)
 };
 
 // dchi2/d(pixel value) = gz: accumulate into dI (workgroup-local copy in LDS)
@@ -648,6 +649,122 @@ __device__ __forceinline__ void adjrow_sym_body(const AdjRowArgs<T>& a, int blk)
 template <typename T, int LOG2N>
 __global__ __launch_bounds__(FftShape<LOG2N>::NT) void k_adjrow_sym(AdjRowArgs<T> a) {
     adjrow_sym_body<T, LOG2N>(a, blockIdx.x);
+}
+
+// Residue form of the adjoint row pass (N >= 1024, Rs' <= 256).  Its outputs are the pixels
+// j = 0..Rs' of the N-point transform of the weighted even extension of two Th rows (real and
+// imaginary parts), so it is the residue adjoint column transform over rows instead of column pairs:
+// the rows are staged in LDS (real, then imaginary parts), one wave per residue class k0 = 0..P/2 of
+// the input frequencies takes the 512-point transform of its decimated sequence, and the waves'
+// terms W_N^(j k0) Y_k0[j] (with the mirror term of residue P - k0) are summed in LDS in residue
+// order.  The pixel adjoints, the central pixel and the dI flush are those of adjrow_sym_body.  Its
+// 5 waves per workgroup (N = 4096) run their transforms without workgroup barriers, which shortens
+// the single workgroup path that the pass's ~100 workgroups take.
+template <typename T, int LOG2N>
+__device__ __forceinline__ void adjrow_sub_body(const AdjRowArgs<T>& a, int blk) {
+    using CS = ResidueShape<LOG2N>;
+    using F = ResidueFft<T>;
+    constexpr int N = CS::N, P = CS::P, NT = CS::NT, NK = CS::NK;
+    constexpr int C = N / 2 + 1;
+    constexpr int PER = (C + NT - 1) / NT;              // staged frequencies per thread
+    constexpr int MOUT = 260;                           // pixels 0..Rs' (<= 256) per residue
+    constexpr size_t STG_BYTES = size_t(N / 2 + N / 16 + 1) * sizeof(T);
+    constexpr size_t EXCH_BYTES = size_t(NK) * F::LDS_REALS * sizeof(T);
+    constexpr size_t SUM_BYTES = size_t(NK) * MOUT * sizeof(cpx<T>);
+    constexpr size_t BYTES = STG_BYTES + EXCH_BYTES > SUM_BYTES ? STG_BYTES + EXCH_BYTES : SUM_BYTES;
+    __shared__ __align__(16) unsigned char smem[BYTES];
+    __shared__ double gIl[ADJROW_LDS_NR];    // this workgroup's share of dchi2/dI (nr <= ADJROW_LDS_NR)
+    __shared__ double red[16];
+    __shared__ double gcen;
+    T* stg = reinterpret_cast<T*>(smem);
+    T* exch = reinterpret_cast<T*>(smem + STG_BYTES);
+    cpx<T>* sums = reinterpret_cast<cpx<T>*>(smem);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int k0 = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int k = tid; k < a.sw.nr; k += NT) gIl[k] = 0.0;   // ordered before use by the barriers below
+    const int ma = 2 * blk, mb = ma + 1;
+    const bool hb = mb < a.nrows;
+    const T* ra = a.Th + size_t(ma) * a.ldTh;
+    const T* rb = ra + a.ldTh;
+    // row ma staged (frequency p at p + p/8: the waves' stride-P gathers spread over the banks), row mb
+    // held in registers until the first gather is done
+    T xb[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int p = tid + q * NT;
+        xb[q] = (hb && p < C) ? rb[p] : T(0);
+        if (p < C) stg[p + (p >> 3)] = ra[p];
+    }
+    __syncthreads();
+    cpx<T> v[8];
+    int sp[8];
+    T wk[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int k = k0 + P * (lane + 64 * r);
+        const int p = k <= N / 2 ? k : N - k;
+        sp[r] = p + (p >> 3);
+        wk[r] = T(ext_weight(k, N));
+        v[r].re = wk[r] * stg[sp[r]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int p = tid + q * NT;
+        if (p < C) stg[p + (p >> 3)] = xb[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r].im = wk[r] * stg[sp[r]];
+    F::run(v, exch + k0 * F::LDS_REALS, lane, a.tw, a.tw512, P, 0);
+    // this residue's terms of pixels j = lane + 64 s
+    const bool mirror = k0 != 0 && 2 * k0 != P;
+    const int src = (64 - lane) & 63;
+    cpx<T> c[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        const int j = lane + 64 * s;
+        const cpx<T> w = a.tw[(j * k0) & (N - 1)];
+        c[s] = cmul(v[s], w);
+        if (mirror) {
+            const T yr = __shfl(v[7 - s].re, src, 64), yi = __shfl(v[7 - s].im, src, 64);
+            const cpx<T> ym = lane == 0 ? v[(8 - s) & 7] : cpx<T>{yr, yi};
+            c[s] = cadd(c[s], cmul(ym, cconj(w)));
+        }
+    }
+    __syncthreads();                   // every wave is done with the staged rows and its exchange region
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        const int j = lane + 64 * s;
+        if (j <= a.Rs) sums[k0 * MOUT + j] = c[s];
+    }
+    __syncthreads();
+    double ginc = 0.0;
+    for (int j = tid; j <= a.Rs; j += NT) {
+        cpx<T> acc = sums[j];
+#pragma unroll
+        for (int k = 1; k < NK; ++k) acc = cadd(acc, sums[k * MOUT + j]);
+        const double cj = mult(j, N);
+        const double gza = cj * double(acc.re), gzb = cj * double(acc.im);
+        if (ma == 0 && j == 0) gcen = gza;      // the central pixel: spread over the workgroup below
+        else ginc += sweep_pixel_adjoint(j, ma, gza, a.sw, a.tan_inc, gIl);
+        if (hb) ginc += sweep_pixel_adjoint(j, mb, gzb, a.sw, a.tan_inc, gIl);
+    }
+    if (ma == 0) {                                 // (uniform per workgroup)
+        __syncthreads();
+        central_adjoint(gcen, a.sw, gIl, tid, NT);
+    }
+    const double b = block_sum_any(ginc, red);    // (its barrier also orders the LDS sums before the flush)
+    if (tid == 0 && b != 0.0) atomic_add_f64(a.ginc, b);
+    for (int k = tid; k < a.sw.nr; k += NT) {      // consecutive bins: coalesced atomics
+        const double x = gIl[k];
+        if (x != 0.0) atomic_add_f64(a.gI + k, x);
+    }
+}
+
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(ResidueShape<LOG2N>::NT) void k_adjrow_sub(AdjRowArgs<T> a) {
+    adjrow_sub_body<T, LOG2N>(a, blockIdx.x);
 }
 
 }  // namespace gl
