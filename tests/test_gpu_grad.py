@@ -230,14 +230,16 @@ def test_gradient_vs_finite_differences():
     assert not bad, bad
 
 
-def test_gradient_with_many_inner_bins():
+@pytest.mark.parametrize("N", [256, 1024])
+def test_gradient_with_many_inner_bins(N):
     """dR = dxy/300 puts ~150 profile bins inside the central pixel, more than
     the 64 threads of the N=256 row passes: the forward central value reads
     bins beyond them from memory and the central pixel's adjoint is spread
-    over the workgroup in several rounds.  Against the torch reference, as in
-    test_gradient_vs_torch_reference; bins 0..iIN+1 get their gradient from
-    the central pixel alone."""
-    N, dxy = 256, 0.01 * arcsec
+    over the workgroup in several rounds.  At N = 1024 the adjoint row pass
+    takes its residue form (320 threads, profile within 255 pixels).  Against
+    the torch reference, as in test_gradient_vs_torch_reference; bins
+    0..iIN+1 get their gradient from the central pixel alone."""
+    dxy = 0.01 * arcsec
     dR = dxy / 300
     d = problem(N=N, n=4000, dxy=dxy, dR=dR, nr=1500)
     iIN = math.floor((dxy / 2 - d['Rmin']) / dR)
@@ -255,12 +257,13 @@ def test_gradient_with_many_inner_bins():
 @pytest.mark.parametrize("geom", [(0.0, 0.0, 0.0, 0.0), (0.4, 0.0, 0.0, 0.0), (0.0, 0.7, 0.0, 0.0),
                                   (0.5, 0.0, 0.02, -0.01)],
                          ids=['all_zero', 'inc_only', 'PA_only', 'PA_zero_with_offset'])
-def test_gradient_at_special_geometry(geom):
+@pytest.mark.parametrize("N", [256, 1024])
+def test_gradient_at_special_geometry(geom, N):
     """PA = 0 skips the rotation and dRA = dDec = 0 the phase factor in the
     kernels; the gradients there (including d/dPA, d/ddRA, d/ddDec at 0) against
-    the torch reference, as in test_gradient_vs_torch_reference.  Offsets in
-    arcsec."""
-    d = problem(N=256, n=4000, dxy=0.01 * arcsec, dR=0.002 * arcsec, nr=400)
+    the torch reference, as in test_gradient_vs_torch_reference, with the full
+    (N = 256) and residue (N = 1024) adjoint transforms.  Offsets in arcsec."""
+    d = problem(N=N, n=4000, dxy=0.01 * arcsec, dR=0.002 * arcsec, nr=400)
     g = np.array(geom, dtype=float)
     g[2:] *= arcsec
     d['geom'] = g
