@@ -305,10 +305,11 @@ void launch_colpass_L(int blocks, hipStream_t s, const ColArgs<T>& a) {
     }
 
 
-// The residue column transforms (k_colpass_sub) when the profile's support
-// fits their 512-row window and N >= 1024 (at N = 512 there is one residue,
-// the whole column, and the workgroup transform is as fast); the full
-// workgroup transform otherwise.
+// The residue transforms (k_colpass_sub, and for the single gradient call
+// k_adjcol_sub and k_adjrow_sub) when the profile's support fits their
+// 512-row window and N >= 1024 (at N = 512 there is one residue, the whole
+// column, and the workgroup transform is as fast); the full workgroup
+// transform otherwise.
 inline bool residue_fits(int L, int Rs) { return L >= 10 && L <= 13 && Rs <= RESIDUE_MAX_RS; }
 
 #define GL_RESIDUE_SWITCH(CALL)                                                  \
