@@ -143,10 +143,7 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
     // bounding box of the workgroup's cells, columns in pairs (the window follows h_index)
     int b0 = live ? (cu0 >> 1) : INT_MAX, b1 = live ? (cu1 >> 1) : -1;
     int b2 = live ? min(cm0, cm1) : INT_MAX, b3 = live ? max(cm0, cm1) : -1;
-    b0 = wave_min(b0);
-    b1 = wave_max(b1);
-    b2 = wave_min(b2);
-    b3 = wave_max(b3);
+    wave_minmax2(b0, b1, b2, b3, lane);
     if (lane == 0) {
         box[wv][0] = b0;
         box[wv][1] = b1;
@@ -234,11 +231,10 @@ __device__ __forceinline__ void sample_grad_sym_body(const SampleArgs<T>& a, con
             }
         }
     }
-    // per-workgroup partials of chi2, dPA, ddRA, ddDec (fixed order)
+    // per-workgroup partials of chi2, dPA, ddRA, ddDec (fixed order; chi2 in the forward kernels' order,
+    // so the gradient call's chi2 equals the forward call's bit for bit)
     c2 = wave_sum(c2);
-    gpa = wave_sum(gpa);
-    gra = wave_sum(gra);
-    gde = wave_sum(gde);
+    wave_sum3(gpa, gra, gde, lane);
     if (lane == 0) {
         red[0][wv] = c2;
         red[1][wv] = gpa;

@@ -52,7 +52,9 @@ enum : int {
     DPP_XOR1 = 0xB1,             // quad_perm [1,0,3,2]: lane l <- l ^ 1
     DPP_XOR2 = 0x4E,             // quad_perm [2,3,0,1]: lane l <- l ^ 2
     DPP_ROW_MIRROR = 0x140,      // lane l <- 15 - l within each row of 16
-    DPP_ROW_HALF_MIRROR = 0x141  // lane l <- 7 - l within each half row of 8
+    DPP_ROW_HALF_MIRROR = 0x141, // lane l <- 7 - l within each half row of 8
+    DPP_ROW_ROR4 = 0x124,        // row rotation by 4 lanes within each row of 16
+    DPP_ROW_ROR8 = 0x128         // row rotation by 8 lanes within each row of 16
 };
 
 template <int CTRL>
@@ -79,6 +81,52 @@ __device__ __forceinline__ double wave_sum(double x) {
     x += dpp_f64<DPP_ROW_HALF_MIRROR>(x);
     x += dpp_f64<DPP_ROW_MIRROR>(x);
     return (lane_f64(x, 0) + lane_f64(x, 16)) + (lane_f64(x, 32) + lane_f64(x, 48));
+}
+
+// Three wave64 sums at once, in a fixed order of their own (not wave_sum's); all 64 lanes active,
+// results in every lane.  The lanes trade halves of the vector: after the pair step even lanes carry
+// the pair sums of (a, b) and odd lanes those of (c, 0), after the quad step lane l carries one
+// quad sum (l mod 4 = 0: a, 1: c, 2: b, 3: 0); then rows of 16 by rotations by 4 and 8 (lanes 4
+// apart carry the same sum) and the rows by two shuffles.  About half the instructions of three
+// wave_sum calls.
+__device__ __forceinline__ void wave_sum3(double& a, double& b, double& c, int lane) {
+    const bool o1 = lane & 1, o2 = lane & 2;
+    double k0 = o1 ? c : a, k1 = o1 ? 0.0 : b;
+    k0 += dpp_f64<DPP_XOR1>(o1 ? a : c);
+    k1 += dpp_f64<DPP_XOR1>(o1 ? b : 0.0);
+    double k = o2 ? k1 : k0;
+    k += dpp_f64<DPP_XOR2>(o2 ? k0 : k1);
+    k += dpp_f64<DPP_ROW_ROR4>(k);
+    k += dpp_f64<DPP_ROW_ROR8>(k);
+    k += __shfl_xor(k, 16, 64);
+    k += __shfl_xor(k, 32, 64);
+    a = lane_f64(k, 0);
+    c = lane_f64(k, 1);
+    b = lane_f64(k, 2);
+}
+
+// Wave64 (min, max, min, max) of four ints at once by the same scheme (lane l mod 4 after the quad
+// step: 0 -> lo0, 1 -> lo1, 2 -> hi0, 3 -> hi1; lanes with l & 2 take maxima); results uniform.
+__device__ __forceinline__ void wave_minmax2(int& lo0, int& hi0, int& lo1, int& hi1, int lane) {
+    const bool o1 = lane & 1, o2 = lane & 2;
+    int k0 = o1 ? lo1 : lo0, k1 = o1 ? hi1 : hi0;
+    k0 = min(k0, dpp_i32<DPP_XOR1>(o1 ? lo0 : lo1));
+    k1 = max(k1, dpp_i32<DPP_XOR1>(o1 ? hi0 : hi1));
+    int k = o2 ? k1 : k0;
+    int r = dpp_i32<DPP_XOR2>(o2 ? k0 : k1);
+    k = o2 ? max(k, r) : min(k, r);
+    r = dpp_i32<DPP_ROW_ROR4>(k);
+    k = o2 ? max(k, r) : min(k, r);
+    r = dpp_i32<DPP_ROW_ROR8>(k);
+    k = o2 ? max(k, r) : min(k, r);
+    r = __shfl_xor(k, 16, 64);
+    k = o2 ? max(k, r) : min(k, r);
+    r = __shfl_xor(k, 32, 64);
+    k = o2 ? max(k, r) : min(k, r);
+    lo0 = __builtin_amdgcn_readlane(k, 0);
+    lo1 = __builtin_amdgcn_readlane(k, 1);
+    hi0 = __builtin_amdgcn_readlane(k, 2);
+    hi1 = __builtin_amdgcn_readlane(k, 3);
 }
 
 // Wave64 minimum / maximum of an int, the same value in every lane (all 64 lanes active).
